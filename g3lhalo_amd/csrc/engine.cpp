@@ -1,0 +1,856 @@
+// engine.cpp -- host side of the gfx950 NNMap / NMapMap engine: context and
+// tables in HBM, HOD sets with device n_bar(z), the plain host-driven
+// cubature and the device-resident batched cubature, behind the C ABI of
+// include/g3l.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/g3l.h"
+#include "kernels.h"
+#include "layout.h"
+#include "physics.h"
+
+using namespace g3l;
+
+namespace {
+
+// Constants of the reference (src/constants.h:15-39, src/Cosmology.h:27-28).
+const double kPi = 3.14159;
+const double kLn10 = std::log(10.0);
+const double kC = 2.99792e5;
+const double kGMsun = 1.327e11;
+const double kMpcKm = 3.086e19;
+
+#define G3L_HIP(expr)                                           \
+  do {                                                          \
+    hipError_t e_ = (expr);                                     \
+    if (e_ != hipSuccess) {                                     \
+      std::fprintf(stderr, "g3l: %s failed: %s (%s:%d)\n", #expr, \
+                   hipGetErrorString(e_), __FILE__, __LINE__);  \
+      return G3L_EHIP;                                          \
+    }                                                           \
+  } while (0)
+
+// Device buffer that only grows.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return G3L_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, want) != hipSuccess) {
+      p = nullptr;
+      return G3L_ENOMEM;
+    }
+    bytes = want;
+    return G3L_OK;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// Pinned host buffer that only grows.
+struct HostBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return G3L_OK;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return G3L_ENOMEM;
+    }
+    bytes = want;
+    return G3L_OK;
+  }
+  void release() {
+    if (p) hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+struct HodSet {
+  g3l_hod h1, h2;
+  double A, eps;
+  int identical;
+};
+
+}  // namespace
+
+struct g3l_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nb = 0;
+  double zmin, zmax, kmin, kmax, mmin, mmax;
+  double H0, Om, OL, rho_crit;
+  double zmin_i, zmax_i, mmin_i, mmax_i;
+  std::vector<double> rho;
+  DevBuf tables;     // g p1 p2 w dwdz rho | hmf plin(+nb) bh conc
+  DevTables T{};
+  std::vector<HodSet> sets;
+  DevBuf nbar;       // [set][2][nb]
+  size_t nbar_sets_cap = 0;
+  // batched engine
+  DevBuf pool, split, lists, jobs, state, args, boxes, counters;
+  HostBuf hcounters;
+  // plain engine
+  DevBuf pgeo, psplit, plist, pjob, parg;
+  HostBuf hgeo, hsplit;
+  uint32_t pcap = 0;
+  hipEvent_t ev[4]{};
+  g3l_batch_stats last{};
+};
+
+namespace {
+
+int kind_of(int stat, int term) { return stat == G3L_NNMAP ? term : 3 + term; }
+
+DevHod to_dev(const g3l_hod &h) { return DevHod{h.f, h.alpha, h.mth, h.sigma, h.mprime, h.beta}; }
+
+const double *nbar_ptr(const g3l_ctx *c, int set, int which) {
+  return c->nbar.as<double>() + ((size_t)set * 2 + which) * c->nb;
+}
+
+// HOD and n_bar selection by type (src/NNMap_Model.cu:437-475); NMapMap uses
+// its one HOD (slot 1).
+TermArgs make_args(const g3l_ctx *c, const g3l_request &r) {
+  const HodSet &s = c->sets[r.hodset];
+  TermArgs a{};
+  a.t1 = r.theta[0];
+  a.t2 = r.theta[1];
+  a.t3 = r.theta[2];
+  a.A = s.A;
+  a.eps = s.eps;
+  if (r.stat == G3L_NMAPMAP) {
+    a.h1 = a.h2 = to_dev(s.h1);
+    a.nb1 = a.nb2 = nbar_ptr(c, r.hodset, 0);
+    a.same = 1;
+  } else {
+    bool a1 = r.type1 == 1, a2 = r.type2 == 1;
+    a.h1 = to_dev(a1 ? s.h1 : s.h2);
+    a.nb1 = nbar_ptr(c, r.hodset, a1 ? 0 : 1);
+    a.h2 = to_dev(a2 ? s.h1 : s.h2);
+    a.nb2 = nbar_ptr(c, r.hodset, a2 ? 0 : 1);
+    a.same = (a1 == a2) || s.identical;
+  }
+  return a;
+}
+
+// Integration box of a term (src/NNMap_Model.cu:313-314, 338-339, 363-364).
+int term_box(const g3l_ctx *c, int term, double *lo, double *hi) {
+  int d = 0;
+  lo[d] = std::log10(0.1);
+  hi[d++] = std::log10(1e6);
+  lo[d] = std::log10(0.1);
+  hi[d++] = std::log10(1e6);
+  lo[d] = 0;
+  hi[d++] = 2 * kPi;
+  for (int i = 0; i < term; ++i) {
+    lo[d] = std::log10(c->mmin_i);
+    hi[d++] = std::log10(c->mmax_i);
+  }
+  lo[d] = c->zmin_i;
+  hi[d++] = c->zmax_i;
+  return d;
+}
+
+// Prefactors (src/NNMap_Model.cu:321-322, 346-347, 371-372;
+// src/NMapMap_Model.cu:188-190, 212-214, 235-237).
+double prefactor(const g3l_ctx *c, int stat, int term, double r) {
+  const double tp = 2 * kPi;
+  double v;
+  if (stat == G3L_NNMAP) v = r * 3 * c->H0 * c->H0 * c->Om / 2 / kC / kC;
+  else v = r * 9 * std::pow(c->H0, 4) * std::pow(c->Om, 2) / 4 / std::pow(kC, 4);
+  for (int i = 0; i < 2 + term; ++i) v = v * kLn10;
+  return v / tp / tp / tp;
+}
+
+int check_request(const g3l_ctx *c, const g3l_request &r) {
+  if (r.stat != G3L_NNMAP && r.stat != G3L_NMAPMAP) return G3L_EINVAL;
+  if (r.term < 1 || r.term > 3) return G3L_EINVAL;
+  if (r.hodset < 0 || r.hodset >= (int)c->sets.size()) return G3L_EINVAL;
+  if (!(r.reltol >= 0) || !std::isfinite(r.theta[0]) || !std::isfinite(r.theta[1]) ||
+      !std::isfinite(r.theta[2]))
+    return G3L_EINVAL;
+  return G3L_OK;
+}
+
+int status_of(uint32_t flags) {
+  if (flags & kJobPoolFull) return G3L_ENOMEM_POOL;
+  if (flags & kJobNonfinite) return G3L_NONFINITE;
+  if (flags & kJobCapped) return G3L_MAXEVAL_HIT;
+  return G3L_OK;
+}
+
+// Slot capacity of one integral: with a budget, the last pop happens while
+// numEval < maxEval, so n <= 1 + (maxEval + pts) / (2 pts).
+uint32_t slot_cap(uint64_t maxeval, int pts, uint32_t unlimited_cap) {
+  if (maxeval == 0) return unlimited_cap;
+  uint64_t c = 2 + (maxeval + (uint64_t)pts) / (2ull * (uint64_t)pts);
+  return (uint32_t)std::min<uint64_t>(c, 1ull << 26);
+}
+
+// ---------------------------------------------------------------------------
+// Batched engine: all jobs refine together; per round one selection launch
+// and one evaluation launch per integrand kind.
+struct EngineJobs {
+  std::vector<JobDesc> desc;
+  std::vector<TermArgs> args;
+  std::vector<double> boxes;  // per job lo[7], hi[7]
+};
+
+int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed) {
+  const uint32_t nj = (uint32_t)E.desc.size();
+  out.assign(nj, JobState{});
+  if (nj == 0) return G3L_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  uint64_t slots = 0;
+  uint64_t per_kind[kNumKinds] = {0};
+  for (auto &d : E.desc) {
+    d.base = (uint32_t)slots;
+    slots += d.cap;
+    per_kind[d.kind] += 2ull * d.cap;
+  }
+  if (slots >= (1ull << 32) - 1) return G3L_ENOMEM;
+  uint64_t list_total = 0;
+  for (int k = 0; k < kNumKinds; ++k) list_total += per_kind[k];
+  int rc;
+  if ((rc = c->pool.ensure(slots * kNumFields * sizeof(double)))) return rc;
+  if ((rc = c->split.ensure(slots))) return rc;
+  if ((rc = c->lists.ensure(list_total * sizeof(EvalItem)))) return rc;
+  if ((rc = c->jobs.ensure(nj * sizeof(JobDesc)))) return rc;
+  if ((rc = c->state.ensure(nj * sizeof(JobState)))) return rc;
+  if ((rc = c->args.ensure(E.args.size() * sizeof(TermArgs)))) return rc;
+  if ((rc = c->boxes.ensure(E.boxes.size() * sizeof(double)))) return rc;
+  if ((rc = c->counters.ensure(64))) return rc;
+  if ((rc = c->hcounters.ensure(64))) return rc;
+  hipStream_t s = c->stream;
+  G3L_HIP(hipMemcpyAsync(c->jobs.p, E.desc.data(), nj * sizeof(JobDesc), hipMemcpyHostToDevice, s));
+  G3L_HIP(hipMemcpyAsync(c->args.p, E.args.data(), E.args.size() * sizeof(TermArgs),
+                         hipMemcpyHostToDevice, s));
+  G3L_HIP(hipMemcpyAsync(c->boxes.p, E.boxes.data(), E.boxes.size() * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  EvalItem *lists[kNumKinds];
+  {
+    uint64_t off = 0;
+    for (int k = 0; k < kNumKinds; ++k) {
+      lists[k] = c->lists.as<EvalItem>() + off;
+      off += per_kind[k];
+    }
+  }
+  uint32_t *dcnt = c->counters.as<uint32_t>();
+  uint32_t *hcnt = c->hcounters.as<uint32_t>();
+
+  RootsLaunch R{};
+  R.jobs = c->jobs.as<JobDesc>();
+  R.boxes = c->boxes.as<double>();
+  R.state = c->state.as<JobState>();
+  R.geo = c->pool.as<double>();
+  R.split = c->split.as<uint8_t>();
+  R.stride = slots;
+  for (int k = 0; k < kNumKinds; ++k) R.lists[k] = lists[k];
+  R.counters = dcnt;
+  R.njobs = nj;
+  G3L_HIP(launch_zero(dcnt, kNumKinds + 1, s));
+  G3L_HIP(launch_roots(R, s));
+
+  EvalLaunch EL{};
+  EL.T = c->T;
+  EL.args = c->args.as<TermArgs>();
+  EL.jobs = c->jobs.as<JobDesc>();
+  EL.geo = c->pool.as<double>();
+  EL.split = c->split.as<uint8_t>();
+  EL.stride = slots;
+  EL.njobs = nj;
+
+  SelectLaunch SL{};
+  SL.jobs = c->jobs.as<JobDesc>();
+  SL.state = c->state.as<JobState>();
+  SL.geo = c->pool.as<double>();
+  SL.split = c->split.as<uint8_t>();
+  SL.stride = slots;
+  for (int k = 0; k < kNumKinds; ++k) {
+    SL.lists[k] = lists[k];
+    SL.list_cap[k] = (uint32_t)std::min<uint64_t>(per_kind[k], 0xffffffffull);
+  }
+  SL.counters = dcnt;
+
+  double eval_ms = 0, select_ms = 0;
+  uint64_t launches = 0;
+  uint32_t rounds = 0;
+  G3L_HIP(hipMemcpyAsync(hcnt, dcnt, (kNumKinds + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  G3L_HIP(hipStreamSynchronize(s));
+  for (;;) {
+    if (timed) G3L_HIP(hipEventRecord(c->ev[0], s));
+    for (int k = 0; k < kNumKinds; ++k) {
+      if (hcnt[k] == 0) continue;
+      EL.list = lists[k];
+      EL.count = (uint32_t)std::min<uint64_t>(hcnt[k], per_kind[k]);
+      G3L_HIP(launch_eval(k, EL, s));
+      ++launches;
+    }
+    if (timed) G3L_HIP(hipEventRecord(c->ev[1], s));
+    G3L_HIP(launch_zero(dcnt, kNumKinds + 1, s));
+    G3L_HIP(launch_select(SL, nj, s));
+    if (timed) G3L_HIP(hipEventRecord(c->ev[2], s));
+    G3L_HIP(hipMemcpyAsync(hcnt, dcnt, (kNumKinds + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           s));
+    G3L_HIP(hipStreamSynchronize(s));
+    if (timed) {
+      float a = 0, b = 0;
+      G3L_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+      G3L_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+      eval_ms += a;
+      select_ms += b;
+    }
+    ++rounds;
+    if (hcnt[kNumKinds] == 0) break;
+  }
+  G3L_HIP(hipMemcpyAsync(out.data(), c->state.p, nj * sizeof(JobState), hipMemcpyDeviceToHost, s));
+  G3L_HIP(hipStreamSynchronize(s));
+  uint64_t evals = 0;
+  for (auto &st : out) evals += st.neval;
+  auto t1 = std::chrono::steady_clock::now();
+  c->last.eval_ms = eval_ms;
+  c->last.select_ms = select_ms;
+  c->last.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  c->last.eval_launches = launches;
+  c->last.evals = evals;
+  c->last.rounds = rounds;
+  return G3L_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plain engine: the reference's driver (src/hcubature.cpp:904-1021) on the
+// host, each batch of regions evaluated by one k_eval launch.
+struct HRegion {
+  double c[kMaxDim], h[kMaxDim];
+  double vol, val, err, key;
+  int split;
+};
+
+double box_volume(int D, const double *h) {
+  double v = 1;
+  for (int i = 0; i < D; ++i) v *= 2 * h[i];
+  return v;
+}
+
+void cut(int D, HRegion &R, HRegion &R2) {
+  int d = R.split;
+  R2 = R;
+  R.h[d] *= 0.5;
+  R.vol *= 0.5;
+  for (int i = 0; i < D; ++i) {
+    R2.c[i] = R.c[i];
+    R2.h[i] = R.h[i];
+  }
+  R2.vol = box_volume(D, R2.h);
+  R.c[d] -= R.h[d];
+  R2.c[d] += R.h[d];
+}
+
+struct HHeap {
+  std::vector<HRegion> a;
+  double tval = 0, terr = 0;
+  void push(const HRegion &r) {
+    tval += r.val;
+    terr += r.err;
+    size_t at = a.size();
+    a.push_back(r);
+    while (at) {
+      size_t parent = (at - 1) / 2;
+      if (r.key <= a[parent].key) break;
+      a[at] = a[parent];
+      at = parent;
+    }
+    a[at] = r;
+  }
+  HRegion pop() {
+    HRegion top = a[0];
+    size_t n = a.size() - 1;
+    a[0] = a[n];
+    a.pop_back();
+    size_t i = 0;
+    for (;;) {
+      size_t child = 2 * i + 1;
+      if (child >= n) break;
+      size_t largest = (a[child].key <= a[i].key) ? i : child;
+      ++child;
+      if (child < n && a[largest].key < a[child].key) largest = child;
+      if (largest == i) break;
+      std::swap(a[i], a[largest]);
+      i = largest;
+    }
+    tval -= top.val;
+    terr -= top.err;
+    return top;
+  }
+};
+
+int plain_ensure(g3l_ctx *c, uint32_t nR) {
+  if (nR <= c->pcap) return G3L_OK;
+  uint32_t cap = std::max<uint32_t>(4096, c->pcap);
+  while (cap < nR) cap *= 2;
+  int rc;
+  if ((rc = c->pgeo.ensure((size_t)cap * kNumFields * sizeof(double)))) return rc;
+  if ((rc = c->psplit.ensure(cap))) return rc;
+  if ((rc = c->plist.ensure((size_t)cap * sizeof(EvalItem)))) return rc;
+  if ((rc = c->hgeo.ensure((size_t)cap * kNumFields * sizeof(double)))) return rc;
+  if ((rc = c->hsplit.ensure(cap))) return rc;
+  std::vector<EvalItem> ident(cap);
+  for (uint32_t i = 0; i < cap; ++i) ident[i] = EvalItem{i, 0};
+  G3L_HIP(hipMemcpy(c->plist.p, ident.data(), cap * sizeof(EvalItem), hipMemcpyHostToDevice));
+  c->pcap = cap;
+  return G3L_OK;
+}
+
+int plain_eval_batch(g3l_ctx *c, int kind, int D, std::vector<HRegion> &R, size_t nR) {
+  int rc = plain_ensure(c, (uint32_t)nR);
+  if (rc) return rc;
+  const size_t cap = c->pcap;
+  double *hg = c->hgeo.as<double>();
+  for (size_t i = 0; i < nR; ++i) {
+    for (int d = 0; d < D; ++d) {
+      hg[(kFieldC + d) * cap + i] = R[i].c[d];
+      hg[(kFieldH + d) * cap + i] = R[i].h[d];
+    }
+    hg[kFieldVol * cap + i] = R[i].vol;
+  }
+  hipStream_t s = c->stream;
+  G3L_HIP(hipMemcpy2DAsync(c->pgeo.p, cap * sizeof(double), hg, cap * sizeof(double),
+                           nR * sizeof(double), kFieldVol + 1, hipMemcpyHostToDevice, s));
+  EvalLaunch EL{};
+  EL.T = c->T;
+  EL.args = c->parg.as<TermArgs>();
+  EL.jobs = c->pjob.as<JobDesc>();
+  EL.list = c->plist.as<EvalItem>();
+  EL.count = (uint32_t)nR;
+  EL.geo = c->pgeo.as<double>();
+  EL.split = c->psplit.as<uint8_t>();
+  EL.stride = cap;
+  EL.njobs = 1;
+  G3L_HIP(launch_eval(kind, EL, s));
+  G3L_HIP(hipMemcpy2DAsync(hg + kFieldVal * cap, cap * sizeof(double),
+                           c->pgeo.as<double>() + kFieldVal * cap, cap * sizeof(double),
+                           nR * sizeof(double), 2, hipMemcpyDeviceToHost, s));
+  G3L_HIP(hipMemcpyAsync(c->hsplit.p, c->psplit.p, nR, hipMemcpyDeviceToHost, s));
+  G3L_HIP(hipStreamSynchronize(s));
+  const uint8_t *hs = c->hsplit.as<uint8_t>();
+  for (size_t i = 0; i < nR; ++i) {
+    R[i].val = hg[kFieldVal * cap + i];
+    R[i].err = hg[kFieldErr * cap + i];
+    R[i].split = hs[i];
+    R[i].key = R[i].err > 0 ? R[i].err : 0;
+  }
+  return G3L_OK;
+}
+
+bool converged(double val, double err, double reltol) {
+  return !(err > 0 && err > std::fabs(val) * reltol);
+}
+
+int plain_cubature(g3l_ctx *c, int kind, const double *lo, const double *hi, uint64_t maxEval,
+                   double reltol, double *val, double *err, uint64_t *neval, uint32_t *rounds,
+                   bool *capped) {
+  const int D = kind_dim(kind);
+  const int np = rule_points(D);
+  std::vector<HRegion> R(2);
+  HRegion root{};
+  for (int i = 0; i < D; ++i) {
+    root.c[i] = 0.5 * (lo[i] + hi[i]);
+    root.h[i] = 0.5 * (hi[i] - lo[i]);
+  }
+  root.vol = box_volume(D, root.h);
+  R[0] = root;
+  uint64_t numEval = 0;
+  uint32_t nb = 0;
+  HHeap heap;
+  int rc = plain_eval_batch(c, kind, D, R, 1);
+  if (rc) return rc;
+  ++nb;
+  heap.push(R[0]);
+  numEval += np;
+  *capped = false;
+  while (numEval < maxEval || !maxEval) {
+    if (converged(heap.tval, heap.terr, reltol)) break;
+    size_t nR = 0;
+    double eval_ = heap.tval, eerr = heap.terr;
+    do {
+      if (nR + 2 > R.size()) R.resize((nR + 2) * 2);
+      R[nR] = heap.pop();
+      eerr -= R[nR].err;
+      cut(D, R[nR], R[nR + 1]);
+      numEval += (uint64_t)np * 2;
+      nR += 2;
+      if (converged(eval_, eerr, reltol)) break;
+    } while (!heap.a.empty() && (numEval < maxEval || !maxEval));
+    if ((rc = plain_eval_batch(c, kind, D, R, nR))) return rc;
+    ++nb;
+    for (size_t i = 0; i < nR; ++i) heap.push(R[i]);
+  }
+  if (maxEval && numEval >= maxEval) *capped = true;
+  double v = 0, e = 0;
+  for (const HRegion &r : heap.a) {
+    v += r.val;
+    e += r.err;
+  }
+  *val = v;
+  *err = e;
+  *neval = numEval;
+  *rounds = nb;
+  return G3L_OK;
+}
+
+// n_bar(z_i) = ln10 * int m n(m,z) (Ncen + Nsat) dlog10 m for every grid z
+// and both HODs of each new set (src/NNMap_Model.cu:244-282), all on the
+// device in one batched run.
+int compute_nbar(g3l_ctx *c, int first, int count) {
+  const int nb = c->nb;
+  size_t need_sets = (size_t)first + count;
+  if (need_sets > c->nbar_sets_cap) {
+    size_t cap = std::max<size_t>(16, c->nbar_sets_cap);
+    while (cap < need_sets) cap *= 2;
+    DevBuf nb2;
+    int rc = nb2.ensure(cap * 2 * nb * sizeof(double));
+    if (rc) return rc;
+    if (c->nbar.p)
+      G3L_HIP(hipMemcpy(nb2.p, c->nbar.p, (size_t)first * 2 * nb * sizeof(double),
+                        hipMemcpyDeviceToDevice));
+    c->nbar.release();
+    c->nbar = nb2;
+    c->nbar_sets_cap = cap;
+  }
+  // run in chunks of sets to bound the pool
+  const int chunk = 64;
+  for (int s0 = first; s0 < first + count; s0 += chunk) {
+    int s1 = std::min(first + count, s0 + chunk);
+    EngineJobs E;
+    for (int s = s0; s < s1; ++s) {
+      for (int which = 0; which < 2; ++which) {
+        for (int i = 0; i < nb; ++i) {
+          JobDesc d{};
+          d.kind = kNbar;
+          d.dim = 1;
+          d.pts = 15;
+          d.args = (int)E.args.size();
+          d.cap = 2048;
+          d.reltol = 1e-4;
+          d.maxeval = 0;
+          TermArgs a{};
+          a.t1 = c->zmin + i * (c->zmax - c->zmin) / nb;
+          a.h1 = a.h2 = to_dev(which == 0 ? c->sets[s].h1 : c->sets[s].h2);
+          E.args.push_back(a);
+          E.desc.push_back(d);
+          double box[2 * kMaxDim] = {0};
+          box[0] = std::log10(c->mmin_i);
+          box[kMaxDim] = std::log10(c->mmax_i);
+          E.boxes.insert(E.boxes.end(), box, box + 2 * kMaxDim);
+        }
+      }
+    }
+    std::vector<JobState> st;
+    g3l_batch_stats keep = c->last;
+    int rc = run_engine(c, E, st, false);
+    c->last = keep;
+    if (rc) return rc;
+    std::vector<double> nbv(st.size());
+    for (size_t j = 0; j < st.size(); ++j) {
+      if (st[j].flags & (kJobPoolFull | kJobNonfinite)) return status_of(st[j].flags);
+      nbv[j] = kLn10 * st[j].val;
+    }
+    G3L_HIP(hipMemcpy(c->nbar.as<double>() + (size_t)s0 * 2 * nb, nbv.data(),
+                      nbv.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return G3L_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+const char *g3l_version(void) { return "g3lhalo_amd 0.1 (gfx950)"; }
+
+int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
+  if (!t || !out || t->nbins < 2 || !t->g || !t->p_lens1 || !t->w || !t->dwdz || !t->hmf ||
+      !t->P_lin || !t->b_h || !t->concentration)
+    return G3L_EINVAL;
+  if (!(t->zmax > t->zmin) || !(t->kmax > t->kmin) || !(t->mmax > t->mmin) || t->mmin <= 0 ||
+      t->kmin <= 0)
+    return G3L_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return G3L_ENODEV;
+  g3l_ctx *c = new (std::nothrow) g3l_ctx();
+  if (!c) return G3L_ENOMEM;
+  const int nb = t->nbins;
+  c->nb = nb;
+  c->zmin = t->zmin; c->zmax = t->zmax; c->kmin = t->kmin; c->kmax = t->kmax;
+  c->mmin = t->mmin; c->mmax = t->mmax;
+  c->H0 = t->H0; c->Om = t->Omega_m; c->OL = t->Omega_Lambda;
+  c->rho_crit = 3 * c->H0 * c->H0 * kMpcKm / 8 / kPi / kGMsun;
+  // integral borders (src/NNMap_Model.cu:46-49)
+  c->zmin_i = c->zmin;
+  c->zmax_i = c->zmax - (c->zmax - c->zmin) / nb;
+  c->mmin_i = c->mmin;
+  c->mmax_i = std::exp(std::log(c->mmin) + std::log(c->mmax / c->mmin) * (nb - 1) / nb);
+  // rho_bar(z) = Omega_m(z) rho_crit (src/NNMap_Model.cu:256, src/Cosmology.cpp:79-87)
+  c->rho.resize(nb);
+  for (int i = 0; i < nb; ++i) {
+    double z = c->zmin + i * (c->zmax - c->zmin) / nb;
+    double E = c->Om * (1 + z) * (1 + z) * (1 + z) + c->OL;
+    c->rho[i] = c->Om * (1 + z) * (1 + z) * (1 + z) / E * c->rho_crit;
+  }
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return G3L_EHIP;
+  }
+  for (auto &e : c->ev) hipEventCreate(&e);
+  const size_t n1 = nb, n2 = (size_t)nb * nb;
+  const size_t total = 6 * n1 + 4 * n2 + n1;
+  if (c->tables.ensure(total * sizeof(double))) {
+    g3l_ctx_destroy(c);
+    return G3L_ENOMEM;
+  }
+  std::vector<double> h(total, 0.0);
+  double *p = h.data();
+  auto put = [&](const double *src, size_t n) {
+    double *at = p;
+    if (src) std::memcpy(p, src, n * sizeof(double));
+    p += n;
+    return (size_t)(at - h.data());
+  };
+  size_t og = put(t->g, n1), op1 = put(t->p_lens1, n1),
+         op2 = put(t->p_lens2 ? t->p_lens2 : t->p_lens1, n1), ow = put(t->w, n1),
+         odw = put(t->dwdz, n1), orho = put(c->rho.data(), n1), ohmf = put(t->hmf, n2),
+         opl = put(t->P_lin, n2 + n1), obh = put(t->b_h, n2), oc = put(t->concentration, n2);
+  // P_lin keeps nb trailing zeros: the look-up has no upper guard
+  // (src/NNMap_Model.cu:826-839), so the last row's overrun reads zeros.
+  std::memset(h.data() + opl + n2, 0, n1 * sizeof(double));
+  if (hipMemcpy(c->tables.p, h.data(), total * sizeof(double), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    g3l_ctx_destroy(c);
+    return G3L_EHIP;
+  }
+  double *d = c->tables.as<double>();
+  DevTables &T = c->T;
+  T.nb = nb;
+  T.zmin = c->zmin; T.zmax = c->zmax; T.kmin = c->kmin; T.kmax = c->kmax;
+  T.mmin = c->mmin; T.mmax = c->mmax;
+  T.inv_dz = nb / (c->zmax - c->zmin);
+  T.g = d + og; T.p1 = d + op1; T.p2 = d + op2; T.w = d + ow; T.dwdz = d + odw;
+  T.rho = d + orho; T.hmf = d + ohmf; T.plin = d + opl; T.bh = d + obh; T.conc = d + oc;
+  *out = c;
+  return G3L_OK;
+}
+
+int g3l_ctx_destroy(g3l_ctx *c) {
+  if (!c) return G3L_EINVAL;
+  hipDeviceSynchronize();
+  c->tables.release();
+  c->nbar.release();
+  c->pool.release(); c->split.release(); c->lists.release(); c->jobs.release();
+  c->state.release(); c->args.release(); c->boxes.release(); c->counters.release();
+  c->hcounters.release();
+  c->pgeo.release(); c->psplit.release(); c->plist.release(); c->pjob.release();
+  c->parg.release(); c->hgeo.release(); c->hsplit.release();
+  for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return G3L_OK;
+}
+
+int g3l_hodset_create_many(g3l_ctx *c, int32_t n, const g3l_hod *hod1, const g3l_hod *hod2,
+                           const double *A, const double *eps, const int32_t *identical,
+                           int32_t *first_id) {
+  if (!c || n <= 0 || !hod1 || !first_id) return G3L_EINVAL;
+  int first = (int)c->sets.size();
+  for (int i = 0; i < n; ++i) {
+    HodSet s;
+    s.h1 = hod1[i];
+    s.h2 = hod2 ? hod2[i] : hod1[i];
+    s.A = A ? A[i] : 0.0;
+    s.eps = eps ? eps[i] : 0.0;
+    s.identical = identical ? identical[i] : 0;
+    for (const g3l_hod *h : {&s.h1, &s.h2})
+      if (!(h->mth > 0) || !(h->sigma > 0) || !(h->mprime > 0)) {
+        c->sets.resize(first);
+        return G3L_EINVAL;
+      }
+    c->sets.push_back(s);
+  }
+  int rc = compute_nbar(c, first, n);
+  if (rc) {
+    c->sets.resize(first);
+    return rc;
+  }
+  *first_id = first;
+  return G3L_OK;
+}
+
+int g3l_hodset_create(g3l_ctx *c, const g3l_hod *hod1, const g3l_hod *hod2, double A,
+                      double epsilon, int32_t identical, int32_t *set_id) {
+  return g3l_hodset_create_many(c, 1, hod1, hod2, &A, &epsilon, &identical, set_id);
+}
+
+int g3l_hodset_densities(g3l_ctx *c, int32_t set, double *rho_bar, double *n_bar1,
+                         double *n_bar2) {
+  if (!c || set < 0 || set >= (int)c->sets.size()) return G3L_EINVAL;
+  if (rho_bar) std::memcpy(rho_bar, c->rho.data(), c->nb * sizeof(double));
+  if (n_bar1) G3L_HIP(hipMemcpy(n_bar1, nbar_ptr(c, set, 0), c->nb * sizeof(double),
+                                hipMemcpyDeviceToHost));
+  if (n_bar2) G3L_HIP(hipMemcpy(n_bar2, nbar_ptr(c, set, 1), c->nb * sizeof(double),
+                                hipMemcpyDeviceToHost));
+  return G3L_OK;
+}
+
+int g3l_hodset_clear(g3l_ctx *c) {
+  if (!c) return G3L_EINVAL;
+  c->sets.clear();
+  return G3L_OK;
+}
+
+int g3l_eval_plain(g3l_ctx *c, const g3l_request *r, g3l_result *out) {
+  if (!c || !r || !out) return G3L_EINVAL;
+  int rc = check_request(c, *r);
+  if (rc) return rc;
+  const int kind = kind_of(r->stat, r->term);
+  TermArgs a = make_args(c, *r);
+  JobDesc d{};
+  d.kind = kind;
+  d.dim = kind_dim(kind);
+  d.pts = rule_points(d.dim);
+  d.args = 0;
+  if ((rc = c->parg.ensure(sizeof(TermArgs)))) return rc;
+  if ((rc = c->pjob.ensure(sizeof(JobDesc)))) return rc;
+  G3L_HIP(hipMemcpy(c->parg.p, &a, sizeof(a), hipMemcpyHostToDevice));
+  G3L_HIP(hipMemcpy(c->pjob.p, &d, sizeof(d), hipMemcpyHostToDevice));
+  double lo[kMaxDim], hi[kMaxDim];
+  term_box(c, r->term, lo, hi);
+  double val = 0, err = 0;
+  uint64_t ne = 0;
+  uint32_t nbt = 0;
+  bool capped = false;
+  rc = plain_cubature(c, kind, lo, hi, r->maxeval, r->reltol, &val, &err, &ne, &nbt, &capped);
+  if (rc) return rc;
+  out->raw = val;
+  out->err = err;
+  out->value = prefactor(c, r->stat, r->term, val);
+  out->neval = ne;
+  out->rounds = nbt;
+  out->status = !std::isfinite(val) || !std::isfinite(err) ? G3L_NONFINITE
+                : capped                                     ? G3L_MAXEVAL_HIT
+                                                             : G3L_OK;
+  return G3L_OK;
+}
+
+int g3l_eval_batch(g3l_ctx *c, const g3l_request *reqs, size_t n, int32_t precision,
+                   g3l_result *out) {
+  if (!c || (!reqs && n) || (!out && n)) return G3L_EINVAL;
+  if (precision != G3L_FP64) return G3L_EINVAL;
+  EngineJobs E;
+  E.desc.reserve(n);
+  E.args.reserve(n);
+  E.boxes.reserve(n * 2 * kMaxDim);
+  for (size_t i = 0; i < n; ++i) {
+    int rc = check_request(c, reqs[i]);
+    if (rc) return rc;
+    const int kind = kind_of(reqs[i].stat, reqs[i].term);
+    JobDesc d{};
+    d.kind = kind;
+    d.dim = kind_dim(kind);
+    d.pts = rule_points(d.dim);
+    d.args = (int)i;
+    d.reltol = reqs[i].reltol;
+    d.maxeval = reqs[i].maxeval;
+    d.cap = slot_cap(d.maxeval, d.pts, 1u << 16);
+    E.desc.push_back(d);
+    E.args.push_back(make_args(c, reqs[i]));
+    double box[2 * kMaxDim] = {0};
+    term_box(c, reqs[i].term, box, box + kMaxDim);
+    E.boxes.insert(E.boxes.end(), box, box + 2 * kMaxDim);
+  }
+  std::vector<JobState> st;
+  int rc = run_engine(c, E, st, true);
+  if (rc) return rc;
+  for (size_t i = 0; i < n; ++i) {
+    out[i].raw = st[i].val;
+    out[i].err = st[i].err;
+    out[i].value = prefactor(c, reqs[i].stat, reqs[i].term, st[i].val);
+    out[i].neval = st[i].neval;
+    out[i].rounds = st[i].rounds + 1;
+    out[i].status = status_of(st[i].flags);
+  }
+  return G3L_OK;
+}
+
+
+int g3l_integrand(g3l_ctx *c, const g3l_request *r, size_t npts, const double *x,
+                  double *f) {
+  if (!c || !r || (npts && (!x || !f)) || npts > 0xffffffffull) return G3L_EINVAL;
+  int rc;
+  int kind;
+  TermArgs a{};
+  if (r->term == 0) {
+    if (r->hodset < 0 || r->hodset >= (int)c->sets.size()) return G3L_EINVAL;
+    kind = kNbar;
+    a.t1 = r->theta[0];
+    const HodSet &s = c->sets[r->hodset];
+    a.h1 = a.h2 = to_dev(r->type1 == 2 ? s.h2 : s.h1);
+  } else {
+    if ((rc = check_request(c, *r))) return rc;
+    kind = kind_of(r->stat, r->term);
+    a = make_args(c, *r);
+  }
+  if (npts == 0) return G3L_OK;
+  const int D = kind_dim(kind);
+  DevBuf dx, df, da;
+  if (dx.ensure(npts * D * sizeof(double)) || df.ensure(npts * sizeof(double)) ||
+      da.ensure(sizeof(TermArgs))) {
+    dx.release(); df.release(); da.release();
+    return G3L_ENOMEM;
+  }
+  rc = G3L_OK;
+  if (hipMemcpy(dx.p, x, npts * D * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(da.p, &a, sizeof(a), hipMemcpyHostToDevice) != hipSuccess ||
+      launch_points(kind, c->T, da.as<TermArgs>(), dx.as<double>(), (uint32_t)npts,
+                    df.as<double>(), c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess ||
+      hipMemcpy(f, df.p, npts * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = G3L_EHIP;
+  dx.release(); df.release(); da.release();
+  return rc;
+}
+
+int g3l_last_batch_stats(g3l_ctx *c, g3l_batch_stats *out) {
+  if (!c || !out) return G3L_EINVAL;
+  *out = c->last;
+  return G3L_OK;
+}
+
+}  // extern "C"

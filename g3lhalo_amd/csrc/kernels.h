@@ -1,0 +1,55 @@
+// kernels.h -- launch descriptors of the gfx950 cubature kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "layout.h"
+#include "physics.h"
+
+namespace g3l {
+
+// Fused point generation + integrand + rule reduction over an evaluation list.
+struct EvalLaunch {
+  DevTables T;
+  const TermArgs *args;
+  const JobDesc *jobs;
+  const EvalItem *list;
+  uint32_t count;
+  double *geo;        // region pool, field f of slot s at geo[f * stride + s]
+  uint8_t *split;
+  uint64_t stride;
+  uint32_t njobs;     // entries naming a slot >= stride or a job >= njobs are skipped
+};
+
+struct RootsLaunch {
+  const JobDesc *jobs;
+  const double *boxes;  // per job: lo[kMaxDim], hi[kMaxDim]
+  JobState *state;
+  double *geo;
+  uint8_t *split;
+  uint64_t stride;
+  EvalItem *lists[kNumKinds];
+  uint32_t *counters;   // [kNumKinds + 1]
+  uint32_t njobs;
+};
+
+struct SelectLaunch {
+  const JobDesc *jobs;
+  JobState *state;
+  double *geo;
+  uint8_t *split;
+  uint64_t stride;
+  EvalItem *lists[kNumKinds];
+  uint32_t *counters;   // per-kind list fill, then [kNumKinds] = active jobs
+  uint32_t list_cap[kNumKinds];
+};
+
+hipError_t launch_eval(int kind, const EvalLaunch &L, hipStream_t s);
+hipError_t launch_roots(const RootsLaunch &L, hipStream_t s);
+hipError_t launch_select(const SelectLaunch &L, uint32_t njobs, hipStream_t s);
+hipError_t launch_zero(uint32_t *p, int n, hipStream_t s);
+int eval_group_size(int kind);
+hipError_t launch_points(int kind, const DevTables &T, const TermArgs *args, const double *x,
+                         uint32_t npts, double *f, hipStream_t s);
+
+}  // namespace g3l

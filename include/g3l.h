@@ -1,0 +1,152 @@
+/*
+ * g3l.h -- C ABI of the MI355X (gfx950) NNMap / NMapMap integration engine.
+ *
+ * Plain pointers and sizes only.  Every function returns a g3l_status; none
+ * calls exit().  Angles are radians, masses M_sun, distances Mpc, as in the
+ * reference.
+ *
+ * The reference has no C ABI: its C++ classes call the vendored integrator
+ * hcubature_v (src/cubature.h:90-94) with integrand callbacks
+ * (src/cubature.h:55-57).  Each entry point below names the reference
+ * interface it replaces.  The C++ classes of include/g3lhalo/ keep the
+ * reference's class API on top of these functions.
+ */
+#ifndef G3L_H
+#define G3L_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum g3l_status {
+  G3L_OK = 0,
+  G3L_EINVAL = 1,        /* bad argument */
+  G3L_ENOMEM = 2,        /* device or host allocation failed */
+  G3L_EHIP = 3,          /* HIP runtime error */
+  G3L_ENOMEM_POOL = 4,   /* an integral ran out of region slots */
+  G3L_MAXEVAL_HIT = 5,   /* warning: the evaluation budget ended the integral
+                            (the reference returns silently in that case) */
+  G3L_NONFINITE = 6,     /* a NaN/Inf reached an integral's totals */
+  G3L_ENODEV = 7         /* no HIP device */
+} g3l_status;
+
+typedef enum g3l_stat { G3L_NNMAP = 0, G3L_NMAPMAP = 1 } g3l_stat;
+
+/* Precision of the integrand evaluation in g3l_eval_batch. */
+typedef enum g3l_precision { G3L_FP64 = 0, G3L_MIXED = 1 } g3l_precision;
+
+typedef struct g3l_ctx g3l_ctx;
+
+/* Lookup tables and binning of NNMap_Model's constructor
+ * (src/NNMap_Model.h:124-128, src/NMapMap_Model.h:102-106): 1-D arrays of
+ * nbins values on z_i = zmin + i (zmax-zmin)/nbins, 2-D arrays z-major,
+ * nbins*nbins values.  For NMapMap pass the one lens distribution as
+ * p_lens1.  The context copies all tables into HBM. */
+typedef struct g3l_tables {
+  int32_t nbins;
+  double zmin, zmax, kmin, kmax, mmin, mmax;
+  const double *g, *p_lens1, *p_lens2, *w, *dwdz;
+  const double *hmf, *P_lin, *b_h, *concentration;
+  double H0, Omega_m, Omega_Lambda; /* Cosmology fields (src/Cosmology.h) */
+} g3l_tables;
+
+/* One HOD: {f, alpha, M_th, sigma, M', beta} -- the order of
+ * HOD::param_arr (src/HOD.cu:15-20). */
+typedef struct g3l_hod {
+  double f, alpha, mth, sigma, mprime, beta;
+} g3l_hod;
+
+/* One adaptive integral of one term (the arguments of
+ * NNMap_Model::NNMap_1h/2h/3h, src/NNMap_Model.h:197-219, and of
+ * NMapMap_Model::NMapMap_1h/2h/3h, src/NMapMap_Model.h:127-150). */
+typedef struct g3l_request {
+  int32_t stat;      /* g3l_stat */
+  int32_t term;      /* 1, 2, 3 */
+  int32_t hodset;    /* from g3l_hodset_create */
+  int32_t type1;     /* NNMap: 1 selects hod1/n_bar1, otherwise hod2/n_bar2 */
+  int32_t type2;
+  int32_t pad;
+  double theta[3];   /* apertures [rad] */
+  double reltol;     /* reference: 1e-3 (NNMap 1h, 2h; NMapMap), 1e-4 (NNMap 3h) */
+  uint64_t maxeval;  /* reference: 5000000; 0 = unlimited */
+} g3l_request;
+
+typedef struct g3l_result {
+  double value;      /* prefactored term, what NNMap_1h etc. return */
+  double raw;        /* the integral itself */
+  double err;        /* its error estimate */
+  uint64_t neval;    /* integrand evaluations */
+  uint32_t rounds;   /* refinement rounds (Gladwell batches) */
+  int32_t status;    /* g3l_status of this integral */
+} g3l_result;
+
+/* Replaces the table/device set-up of NNMap_Model::NNMap_Model and
+ * NMapMap_Model::NMapMap_Model (src/NNMap_Model.cu:15-101,
+ * src/NMapMap_Model.cu:14-79). */
+int g3l_ctx_create(const g3l_tables *tables, g3l_ctx **out);
+int g3l_ctx_destroy(g3l_ctx *ctx);
+
+/* Replaces NNMap_Model::updateHODs/updateDensities
+ * (src/NNMap_Model.cu:237-297) and NMapMap_Model::updateHOD
+ * (src/NMapMap_Model.cu:116-167): stores the HOD pair and computes rho_bar(z),
+ * n_bar1(z), n_bar2(z) on the device.  identical != 0 marks hod1 and hod2 as
+ * one object, which selects the same-type NsatNsat branch
+ * (src/HOD.cu:81-83) for every type pair. */
+int g3l_hodset_create(g3l_ctx *ctx, const g3l_hod *hod1, const g3l_hod *hod2, double A,
+                      double epsilon, int32_t identical, int32_t *set_id);
+/* n HOD pairs at once; all n_bar integrals refine together on the device. */
+int g3l_hodset_create_many(g3l_ctx *ctx, int32_t n, const g3l_hod *hod1, const g3l_hod *hod2,
+                           const double *A, const double *epsilon, const int32_t *identical,
+                           int32_t *first_id);
+/* Reads rho_bar, n_bar1, n_bar2 (nbins each; any pointer may be NULL). */
+int g3l_hodset_densities(g3l_ctx *ctx, int32_t set_id, double *rho_bar, double *n_bar1,
+                         double *n_bar2);
+/* Drops all HOD sets of the context. */
+int g3l_hodset_clear(g3l_ctx *ctx);
+
+/* Plain form: one adaptive integral, host-driven exactly as hcubature_v
+ * (src/hcubature.cpp:904-1021) -- the region heap on the host, every
+ * Gladwell batch evaluated by one gfx950 kernel launch.  Replaces one call
+ * of NNMap_Model::NNMap_1h/2h/3h or NMapMap_Model::NMapMap_1h/2h/3h. */
+int g3l_eval_plain(g3l_ctx *ctx, const g3l_request *req, g3l_result *out);
+
+/* Batched form: n independent integrals refine together from one HBM region
+ * pool, selection and cuts on the device.  Replaces a loop of the calls
+ * above, e.g. NNMap_Model::calculateAll (src/NNMap_Model.cu:382-415). */
+int g3l_eval_batch(g3l_ctx *ctx, const g3l_request *reqs, size_t n, int32_t precision,
+                   g3l_result *out);
+
+/* The integrand of a request at npts points x[npts][dim] (dim 5/6/7 for
+ * terms 1/2/3, coordinates log10 l1, log10 l2, phi, log10 m..., z), written
+ * to f[npts].  term = 0 selects the n_bar integrand (dim 1, x = log10 m,
+ * theta[0] = z, type1 = HOD slot 1 or 2 of the set).  Replaces the
+ * reference's integrand callbacks NNM::integrand_1halo/2halo/3halo,
+ * NMM::integrand_1halo/2halo/3halo and integrand_nbar
+ * (src/NNMap_Model.h:336,347,446; src/NMapMap_Model.h:247,327,411;
+ * src/generalFunctions.h:127). */
+int g3l_integrand(g3l_ctx *ctx, const g3l_request *req, size_t npts, const double *x,
+                  double *f);
+
+/* Optional timing of the last g3l_eval_batch: device time of the dominant
+ * (evaluation) kernels, total kernel count and algorithmic evaluations. */
+typedef struct g3l_batch_stats {
+  double eval_ms;        /* summed duration of the evaluation kernels */
+  double select_ms;      /* summed duration of the selection kernels */
+  double wall_ms;        /* host wall time of the call */
+  uint64_t eval_launches;
+  uint64_t evals;        /* integrand evaluations */
+  uint32_t rounds;
+  uint32_t pad;
+} g3l_batch_stats;
+int g3l_last_batch_stats(g3l_ctx *ctx, g3l_batch_stats *out);
+
+/* Version string of this build. */
+const char *g3l_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* G3L_H */

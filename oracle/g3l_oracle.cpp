@@ -1,0 +1,1074 @@
+/*
+ * g3l_oracle.cpp -- CPU oracle (FP64, OpenMP) for the NNMap / NMapMap path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * What pins each part (tests/test_oracle_*.py):
+ *   cubature driver, GM 7/5 rule, GK15 rule, heap, Gladwell batching
+ *       -> bit-identical val, err, evaluation count and batch-size trace
+ *          against the reference's own hcubature.cpp compiled from
+ *          src/hcubature.cpp into oracle/_ref (when /root/reference exists),
+ *          on the Genz families and on every physics integrand below;
+ *          closed-form integrals of the Genz families; exactness of the
+ *          7th/5th-degree rules on monomials; GK15 exactness on degree <=22.
+ *   SiCi            -> scipy.special.sici, Si(pi), Ci(1), continuity at x = 4.
+ *   apertureFilter, Ncen, Nsat, NsatNsat -> closed forms with scipy erf.
+ *   u_NFW           -> brute-force Fourier transform of the truncated NFW
+ *                      profile; u -> 1 as k -> 0.
+ *   F, Bi_lin       -> closed form including the k2/k2 quirk; asymmetry.
+ *   table indices   -> synthetic tables whose entries encode their index.
+ *   term integrands -> recomposition from the pinned primitives in numpy at
+ *                      random points (tests/test_oracle_integrands.py).
+ *   n_bar(z)        -> scipy quadrature of the same nearest-index integrand.
+ * The reference's integrand sources (.cu) need CUDA headers that this image
+ * lacks, so they are not compiled here; the integrands are pinned by the
+ * checks above rather than by a run of the reference.
+ */
+#include "g3l_oracle.h"
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+/* Constants of the reference (src/constants.h:15-39, src/Cosmology.h:27-28). */
+const double kPi = 3.14159;
+const double kLn10 = std::log(10.0);
+const double kC = 2.99792e5;
+const double kGMsun = 1.327e11;
+const double kMpcKm = 3.086e19;
+
+/* ---------------------------------------------------------------------- */
+/* Adaptive cubature (restatement of src/hcubature.cpp:115-1067)          */
+/* ---------------------------------------------------------------------- */
+
+const unsigned kMaxDim = 16;
+
+struct Region {
+  double c[kMaxDim];
+  double h[kMaxDim];
+  double vol;
+  double val, err;
+  double key;       /* errmax = max(0, err) (src/hcubature.cpp:119-126) */
+  unsigned split;
+};
+
+/* Product of full widths in dimension order (src/hcubature.cpp:134-141). */
+double box_volume(unsigned dim, const double *h) {
+  double v = 1;
+  for (unsigned i = 0; i < dim; ++i) v *= 2 * h[i];
+  return v;
+}
+
+/* Split along R.split: R becomes the lower half, R2 the upper half.  The
+ * lower half's volume is halved, the upper half's is recomputed
+ * (src/hcubature.cpp:206-218). */
+void cut(unsigned dim, Region &R, Region &R2) {
+  unsigned d = R.split;
+  R2 = R;
+  R.h[d] *= 0.5;
+  R.vol *= 0.5;
+  for (unsigned i = 0; i < dim; ++i) { R2.c[i] = R.c[i]; R2.h[i] = R.h[i]; }
+  R2.vol = box_volume(dim, R2.h);
+  R.c[d] -= R.h[d];
+  R2.c[d] += R.h[d];
+}
+
+unsigned points_per_region(unsigned dim) {
+  if (dim == 1) return 15;
+  return 1 + 2 * (2 * dim) + 2 * dim * (dim - 1) + (1u << dim);
+}
+
+/* Genz-Malik point set, in the order of src/hcubature.cpp:343-414,490-502:
+ * centre; per dimension c-l2h, c+l2h, c-l4h, c+l4h; per pair i<j the four
+ * sign patterns (-,-) (+,-) (+,+) (-,+) at l4h; the 2^d corners at l5h in
+ * reflected Gray-code order starting from all '+'. */
+const double kL2 = 0.3585685828003180919906451539079374954541;
+const double kL4 = 0.9486832980505137995996680633298155601160;
+const double kL5 = 0.6882472016116852977216287342936235251269;
+
+void gm_points(unsigned dim, const Region &R, double *pts) {
+  double r2[kMaxDim], r4[kMaxDim], r5[kMaxDim], p[kMaxDim];
+  for (unsigned i = 0; i < dim; ++i) {
+    r2[i] = R.h[i] * kL2;
+    r4[i] = R.h[i] * kL4;
+    r5[i] = R.h[i] * kL5;
+    p[i] = R.c[i];
+  }
+  size_t n = 0;
+  auto emit = [&]() { std::memcpy(pts + n * dim, p, sizeof(double) * dim); ++n; };
+  emit();
+  for (unsigned i = 0; i < dim; ++i) {
+    p[i] = R.c[i] - r2[i]; emit();
+    p[i] = R.c[i] + r2[i]; emit();
+    p[i] = R.c[i] - r4[i]; emit();
+    p[i] = R.c[i] + r4[i]; emit();
+    p[i] = R.c[i];
+  }
+  for (unsigned i = 0; i + 1 < dim; ++i) {
+    for (unsigned j = i + 1; j < dim; ++j) {
+      double lo_i = R.c[i] - r4[i], hi_i = R.c[i] + r4[i];
+      double lo_j = R.c[j] - r4[j], hi_j = R.c[j] + r4[j];
+      p[i] = lo_i; p[j] = lo_j; emit();
+      p[i] = hi_i; p[j] = lo_j; emit();
+      p[i] = hi_i; p[j] = hi_j; emit();
+      p[i] = lo_i; p[j] = hi_j; emit();
+      p[i] = R.c[i]; p[j] = R.c[j];
+    }
+  }
+  for (unsigned q = 0; q < (1u << dim); ++q) {
+    unsigned signs = q ^ (q >> 1);
+    for (unsigned i = 0; i < dim; ++i)
+      p[i] = ((signs >> i) & 1u) ? R.c[i] - r5[i] : R.c[i] + r5[i];
+    emit();
+  }
+}
+
+/* Rule sums and split choice (src/hcubature.cpp:512-577, 600-606). */
+void gm_reduce(unsigned dim, Region &R, const double *v) {
+  const double w2 = 980. / 6561., w4 = 200. / 19683.;
+  const double e2 = 245. / 486., e4 = 25. / 729.;
+  const double ratio = (kL2 * kL2) / (kL4 * kL4);
+  const int d = (int)dim;
+  const double w1 = (double)(12824 - 9120 * d + 400 * d * d) / (double)19683;
+  const double w3 = (double)(1820 - 400 * d) / (double)19683;
+  const double w5 = (double)6859 / (double)19683 / (double)(1u << dim);
+  const double e1 = (double)(729 - 950 * d + 50 * d * d) / (double)729;
+  const double e3 = (double)(265 - 100 * d) / (double)1458;
+
+  double f0 = v[0];
+  double s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+  double diff[kMaxDim];
+  size_t k0 = 1;
+  for (unsigned k = 0; k < dim; ++k) {
+    double a = v[k0 + 4 * k], b = v[k0 + 4 * k + 1];
+    double c = v[k0 + 4 * k + 2], e = v[k0 + 4 * k + 3];
+    s2 += a + b;
+    s3 += c + e;
+    diff[k] = std::fabs(a + b - 2 * f0 - ratio * (c + e - 2 * f0));
+  }
+  k0 += 4 * dim;
+  for (unsigned k = 0; k < 2 * dim * (dim - 1); ++k) s4 += v[k0 + k];
+  k0 += 2 * dim * (dim - 1);
+  for (unsigned k = 0; k < (1u << dim); ++k) s5 += v[k0 + k];
+
+  double r7 = R.vol * (w1 * f0 + w2 * s2 + w3 * s3 + w4 * s4 + w5 * s5);
+  double r5 = R.vol * (e1 * f0 + e2 * s2 + e3 * s3 + e4 * s4);
+  R.val = r7;
+  R.err = std::fabs(r5 - r7);
+  double best = 0;
+  unsigned arg = 0;
+  for (unsigned i = 0; i < dim; ++i)
+    if (diff[i] > best) { best = diff[i]; arg = i; }
+  R.split = arg;
+}
+
+/* 1-D Gauss-Kronrod 7/15 (src/hcubature.cpp:620-765). */
+const double kXgk[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                        0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                        0.586087235467691130294144838258730, 0.405845151377397166906606412076961,
+                        0.207784955007898467600689403773245, 0.000000000000000000000000000000000};
+const double kWg[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                       0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+const double kWgk[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                        0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                        0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                        0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+
+void gk_points(const Region &R, double *pts) {
+  double c = R.c[0], h = R.h[0];
+  size_t n = 0;
+  pts[n++] = c;
+  for (int j = 0; j < 3; ++j) {
+    double w = h * kXgk[2 * j + 1];
+    pts[n++] = c - w;
+    pts[n++] = c + w;
+  }
+  for (int j = 0; j < 4; ++j) {
+    double w = h * kXgk[2 * j];
+    pts[n++] = c - w;
+    pts[n++] = c + w;
+  }
+}
+
+void gk_reduce(Region &R, const double *v) {
+  double h = R.h[0];
+  double rg = v[0] * kWg[3];
+  double rk = v[0] * kWgk[7];
+  double rabs = std::fabs(rk);
+  size_t n = 1;
+  for (int j = 0; j < 3; ++j) {
+    int j2 = 2 * j + 1;
+    double s = v[n] + v[n + 1];
+    rg += kWg[j] * s;
+    rk += kWgk[j2] * s;
+    rabs += kWgk[j2] * (std::fabs(v[n]) + std::fabs(v[n + 1]));
+    n += 2;
+  }
+  for (int j = 0; j < 4; ++j) {
+    int j2 = 2 * j;
+    rk += kWgk[j2] * (v[n] + v[n + 1]);
+    rabs += kWgk[j2] * (std::fabs(v[n]) + std::fabs(v[n + 1]));
+    n += 2;
+  }
+  R.val = rk * h;
+  double mean = rk * 0.5;
+  double rasc = kWgk[7] * std::fabs(v[0] - mean);
+  n = 1;
+  for (int j = 0; j < 3; ++j) {
+    int j2 = 2 * j + 1;
+    rasc += kWgk[j2] * (std::fabs(v[n] - mean) + std::fabs(v[n + 1] - mean));
+    n += 2;
+  }
+  for (int j = 0; j < 4; ++j) {
+    int j2 = 2 * j;
+    rasc += kWgk[j2] * (std::fabs(v[n] - mean) + std::fabs(v[n + 1] - mean));
+    n += 2;
+  }
+  double err = std::fabs(rk - rg) * h;
+  rabs *= h;
+  rasc *= h;
+  if (rasc != 0 && err != 0) {
+    double scale = std::pow((200 * err / rasc), 1.5);
+    err = (scale < 1) ? rasc * scale : rasc;
+  }
+  if (rabs > DBL_MIN / (50 * DBL_EPSILON)) {
+    double min_err = 50 * DBL_EPSILON * rabs;
+    if (min_err > err) err = min_err;
+  }
+  R.err = err;
+  R.split = 0;
+}
+
+/* Evaluate nR regions in one integrand call (src/hcubature.cpp:285-294). */
+int eval_batch(unsigned dim, std::vector<Region> &R, size_t nR, orc_integrand_v f, void *fdata,
+               std::vector<double> &pts, std::vector<double> &vals) {
+  unsigned np = points_per_region(dim);
+  pts.resize(nR * np * dim);
+  vals.resize(nR * np);
+  for (size_t r = 0; r < nR; ++r) {
+    if (dim == 1) gk_points(R[r], pts.data() + r * np);
+    else gm_points(dim, R[r], pts.data() + r * np * dim);
+  }
+  if (f(dim, nR * np, pts.data(), fdata, 1, vals.data())) return 1;
+  for (size_t r = 0; r < nR; ++r) {
+    if (dim == 1) gk_reduce(R[r], vals.data() + r * np);
+    else gm_reduce(dim, R[r], vals.data() + r * np);
+    R[r].key = R[r].err > 0 ? R[r].err : 0;
+  }
+  return 0;
+}
+
+/* Binary max-heap on errmax with running totals (src/hcubature.cpp:772-890). */
+struct Heap {
+  std::vector<Region> a;
+  double tval = 0, terr = 0;
+
+  void push(const Region &r) {
+    tval += r.val;
+    terr += r.err;
+    size_t at = a.size();
+    a.push_back(r);
+    while (at) {
+      size_t parent = (at - 1) / 2;
+      if (r.key <= a[parent].key) break;
+      a[at] = a[parent];
+      at = parent;
+    }
+    a[at] = r;
+  }
+
+  Region pop() {
+    Region top = a[0];
+    size_t n = a.size() - 1;
+    a[0] = a[n];
+    a.pop_back();
+    size_t i = 0;
+    for (;;) {
+      size_t child = 2 * i + 1;
+      if (child >= n) break;
+      size_t largest = (a[child].key <= a[i].key) ? i : child;
+      ++child;
+      if (child < n && a[largest].key < a[child].key) largest = child;
+      if (largest == i) break;
+      Region t = a[i];
+      a[i] = a[largest];
+      a[largest] = t;
+      i = largest;
+    }
+    tval -= top.val;
+    terr -= top.err;
+    return top;
+  }
+};
+
+/* Convergence test for one integrand (src/converged.h:10-14). */
+bool converged(double val, double err, double reqAbs, double reqRel) {
+  return !(err > reqAbs && err > std::fabs(val) * reqRel);
+}
+
+/* Driver with Gladwell batching (src/hcubature.cpp:904-1021, 1023-1067). */
+int cubature(orc_integrand_v f, void *fdata, unsigned dim, const double *xmin,
+             const double *xmax, size_t maxEval, double reqAbs, double reqRel, double *val,
+             double *err) {
+  if (dim == 0 || dim > kMaxDim) return 1;
+  unsigned np = points_per_region(dim);
+  std::vector<Region> R(2);
+  std::vector<double> pts, vals;
+  Region root;
+  for (unsigned i = 0; i < dim; ++i) {
+    root.c[i] = 0.5 * (xmin[i] + xmax[i]);
+    root.h[i] = 0.5 * (xmax[i] - xmin[i]);
+  }
+  root.vol = box_volume(dim, root.h);
+  root.split = 0;
+  R[0] = root;
+  size_t numEval = 0;
+  Heap heap;
+  if (eval_batch(dim, R, 1, f, fdata, pts, vals)) return 1;
+  heap.push(R[0]);
+  numEval += np;
+
+  while (numEval < maxEval || !maxEval) {
+    if (converged(heap.tval, heap.terr, reqAbs, reqRel)) break;
+    size_t nR = 0;
+    double eval_ = heap.tval, eerr = heap.terr;
+    do {
+      if (nR + 2 > R.size()) R.resize((nR + 2) * 2);
+      R[nR] = heap.pop();
+      eerr -= R[nR].err;
+      cut(dim, R[nR], R[nR + 1]);
+      numEval += (size_t)np * 2;
+      nR += 2;
+      if (converged(eval_, eerr, reqAbs, reqRel)) break;
+    } while (!heap.a.empty() && (numEval < maxEval || !maxEval));
+    if (eval_batch(dim, R, nR, f, fdata, pts, vals)) return 1;
+    for (size_t i = 0; i < nR; ++i) heap.push(R[i]);
+  }
+  double v = 0, e = 0;
+  for (const Region &r : heap.a) {
+    v += r.val;
+    e += r.err;
+  }
+  *val = v;
+  *err = e;
+  return 0;
+}
+
+/* Counting wrapper around an integrand: npts per call = the batch trace. */
+struct Counted {
+  orc_integrand_v f;
+  void *fdata;
+  orc_stats *st;
+  uint32_t *trace;
+  uint32_t cap;
+};
+
+int counted_cb(unsigned ndim, size_t npt, const double *x, void *fd, unsigned fdim, double *fv) {
+  Counted *c = (Counted *)fd;
+  if (c->st) {
+    c->st->neval += npt;
+    if (c->trace && c->st->trace_len < c->cap) c->trace[c->st->trace_len++] = (uint32_t)npt;
+    c->st->ncalls++;
+  }
+  return c->f(ndim, npt, x, c->fdata, fdim, fv);
+}
+
+int run_cubature(orc_hcubature_v_fn hcub, orc_integrand_v f, void *fdata, unsigned dim,
+                 const double *lo, const double *hi, size_t maxEval, double reltol, double *val,
+                 double *err, orc_stats *st, uint32_t *trace, uint32_t cap) {
+  Counted c{f, fdata, st, trace, cap};
+  if (st) std::memset(st, 0, sizeof(*st));
+  /* norm = ERROR_L1 as in the reference's call sites, irrelevant for fdim 1 */
+  if (hcub) return hcub(1, counted_cb, &c, dim, lo, hi, maxEval, 0, reltol, 3, val, err);
+  return orc_hcubature_v(1, counted_cb, &c, dim, lo, hi, maxEval, 0, reltol, 3, val, err);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Physics primitives                                                     */
+/* ---------------------------------------------------------------------- */
+
+struct Hod {
+  double f, alpha, mth, sigma, mprime, beta;
+};
+
+Hod hod_from(const double *p) { return Hod{p[0], p[1], p[2], p[3], p[4], p[5]}; }
+
+/* Rational approximations of Si and Ci (src/generalFunctions.cu:44-87). */
+void sici(double x, double &si, double &ci) {
+  double x2 = x * x;
+  double x4 = x2 * x2;
+  double x6 = x2 * x4;
+  double x8 = x4 * x4;
+  double x10 = x8 * x2;
+  double x12 = x6 * x6;
+  double x14 = x12 * x2;
+  if (x < 4) {
+    double a = 1 - 4.54393409816329991e-2 * x2 + 1.15457225751016682e-3 * x4 -
+               1.41018536821330254e-5 * x6 + 9.43280809438713025e-8 * x8 -
+               3.53201978997168357e-10 * x10 + 7.08240282274875911e-13 * x12 -
+               6.05338212010422477e-16 * x14;
+    double b = 1 + 1.01162145739225565e-2 * x2 + 4.99175116169755106e-5 * x4 +
+               1.55654986308745614e-7 * x6 + 3.28067571055789734e-10 * x8 +
+               4.5049097575386581e-13 * x10 + 3.21107051193712168e-16 * x12;
+    si = x * a / b;
+    double gamma = 0.5772156649;
+    a = -0.25 + 7.51851524438898291e-3 * x2 - 1.27528342240267686e-4 * x4 +
+        1.05297363846239184e-6 * x6 - 4.68889508144848019e-9 * x8 +
+        1.06480802891189243e-11 * x10 - 9.93728488857585407e-15 * x12;
+    b = 1 + 1.1592605689110735e-2 * x2 + 6.72126800814254432e-5 * x4 +
+        2.55533277086129636e-7 * x6 + 6.97071295760958946e-10 * x8 +
+        1.38536352772778619e-12 * x10 + 1.89106054713059759e-15 * x12 +
+        1.39759616731376855e-18 * x14;
+    ci = gamma + std::log(x) + x2 * a / b;
+  } else {
+    double x16 = x8 * x8;
+    double x18 = x16 * x2;
+    double x20 = x10 * x10;
+    double cx = std::cos(x);
+    double sx = std::sin(x);
+    double f = (1 + 7.44437068161936700618e2 / x2 + 1.96396372895146869801e5 / x4 +
+                2.37750310125431834034e7 / x6 + 1.43073403821274636888e9 / x8 +
+                4.33736238870432522765e10 / x10 + 6.40533830574022022911e11 / x12 +
+                4.20968180571076940208e12 / x14 + 1.00795182980368574617e13 / x16 +
+                4.94816688199951963482e12 / x18 - 4.94701168645415959931e11 / x20) /
+               (1 + 7.46437068161927678031e2 / x2 + 1.97865247031583951450e5 / x4 +
+                2.41535670165126845144e7 / x6 + 1.47478952192985464958e9 / x8 +
+                4.58595115847765779830e10 / x10 + 7.08501308149515401563e11 / x12 +
+                5.06084464593475076774e12 / x14 + 1.43468549171581016479e13 / x16 +
+                1.11535493509914254097e13 / x18) /
+               x;
+    double g = (1 + 8.1359520115168615e2 / x2 + 2.35239181626478200e5 / x4 +
+                3.12557570795778731e7 / x6 + 2.06297595146763354e9 / x8 +
+                6.83052205423625007e10 / x10 + 1.09049528450362786e12 / x12 +
+                7.57664583257834349e12 / x14 + 1.81004487464664575e13 / x16 +
+                6.43291613143049485e12 / x18 - 1.36517137670871689e12 / x20) /
+               (1 + 8.19595201151451564e2 / x2 + 2.40036752835578777e5 / x4 +
+                3.26026661647090822e7 / x6 + 2.23355543278099360e9 / x8 +
+                7.87465017341829930e10 / x10 + 1.39866710696414565e12 / x12 +
+                1.17164723371736605e13 / x14 + 4.01839087307656620e13 / x16 +
+                3.99653257887490811e13 / x18) /
+               x2;
+    si = 0.5 * M_PI - f * cx - g * sx;
+    ci = f * sx - g * cx;
+  }
+}
+
+/* U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42). */
+inline double aperture(double eta) { return 0.5 * eta * eta * std::exp(-0.5 * eta * eta); }
+
+/* HOD occupations (src/HOD.cu:35-94). */
+inline double ncen(const Hod &h, double m) {
+  return 0.5 * h.alpha * (1 + std::erf(std::log(m / h.mth) / h.sigma / 1.414213562));
+}
+inline double nsat(const Hod &h, double m) {
+  return 0.5 * (1 + std::erf(std::log(m / h.mth) / h.sigma / 1.414213562)) *
+         std::pow(m / h.mprime, h.beta);
+}
+inline double nsatnsat(const Hod &h1, const Hod &h2, bool same, double A, double eps, double m) {
+  const double scale1 = 1;
+  double ns1 = nsat(h1, m);
+  if (same) return (ns1 + scale1 * scale1 - 1) * ns1;
+  double ns2 = nsat(h2, m);
+  double r = ns1 * ns2 + A * std::pow(m, eps) * std::sqrt(ns1 * ns2);
+  if (r < 0) return 0;
+  return r;
+}
+
+inline double fkernel(double k1, double k2, double c) {
+  return 0.7143 + 0.2857 * c * c + 0.5 * c * (k1 / k2 + k2 / k2);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Model                                                                  */
+/* ---------------------------------------------------------------------- */
+
+struct Model {
+  int nb;
+  double zmin, zmax, kmin, kmax, mmin, mmax;
+  std::vector<double> g, p1, p2, w, dwdz, hmf, plin, bh, conc;
+  double H0, Om, OL, rho_crit;
+  double zmin_i, zmax_i, mmin_i, mmax_i;
+  std::vector<double> rho_bar, nbar1, nbar2;
+  Hod hod1{}, hod2{};
+  bool identical = false;
+  double A = 0, eps = 0;
+
+  /* Nearest-index look-ups: round() for the integrands and Bi_lin,
+   * int() truncation inside u_NFW and r_200 (src/generalFunctions.cu:11-12,35). */
+  inline int zix_round(double z) const { return (int)std::round((z - zmin) * nb / (zmax - zmin)); }
+  inline int mix_round(double m) const {
+    return (int)std::round(std::log(m / mmin) * nb / std::log(mmax / mmin));
+  }
+  inline int kix_round(double k) const {
+    return (int)std::round(std::log(k / kmin) * nb / std::log(kmax / kmin));
+  }
+
+  /* P(k) look-up (src/NNMap_Model.cu:819-839): below kmin linear in k; no
+   * upper guard, so an index past the row reads the next row (P_lin is
+   * stored with nb trailing zeros so the last row's overrun stays inside). */
+  inline double pk(int zix, int kix, double k) const {
+    if (kix >= 0) return plin[(size_t)zix * nb + kix];
+    return plin[(size_t)zix * nb] / kmin * k;
+  }
+
+  double r200(double m, double z) const {
+    int zi = int((z - zmin) * nb / (zmax - zmin));
+    return std::pow(0.239 * m / rho_bar[zi] / 200, 1. / 3.);
+  }
+
+  /* Truncated-NFW Fourier profile (src/generalFunctions.cu:6-30). */
+  double unfw(double k, double m, double z, double f) const {
+    int zi = int((z - zmin) * nb / (zmax - zmin));
+    int mi = int(std::log(m / mmin) * nb / std::log(mmax / mmin));
+    double c = f * conc[(size_t)zi * nb + mi];
+    double arg1 = k * r200(m, z) / c;
+    double arg2 = arg1 * (1 + c);
+    double si1, ci1, si2, ci2;
+    sici(arg1, si1, ci1);
+    sici(arg2, si2, ci2);
+    double term1 = std::sin(arg1) * (si2 - si1);
+    double term2 = std::cos(arg1) * (ci2 - ci1);
+    double term3 = -std::sin(arg1 * c) / arg2;
+    double F = std::log(1. + c) - c / (1. + c);
+    return (term1 + term2 + term3) / F;
+  }
+
+  /* Tree-level bispectrum (src/generalFunctions.cu:94-147). */
+  double bilin(double k1, double k2, double cosphi, double z) const {
+    double k3 = std::sqrt(k1 * k1 + k2 * k2 + 2 * k1 * k2 * cosphi);
+    double c12 = cosphi;
+    double c13 = (k2 * k2 - k3 * k3 - k1 * k1) / (2 * k1 * k3);
+    double c23 = (k1 * k1 - k2 * k2 - k3 * k3) / (2 * k3 * k2);
+    int zi = zix_round(z);
+    double P1 = pk(zi, kix_round(k1), k1);
+    double P2 = pk(zi, kix_round(k2), k2);
+    double P3 = pk(zi, kix_round(k3), k3);
+    return 2 * (fkernel(k1, k2, c12) * P1 * P2 + fkernel(k1, k3, c13) * P1 * P3 +
+                fkernel(k2, k3, c23) * P2 * P3);
+  }
+
+  /* G_g and G_gg (src/NNMap_Model.cu:1071-1108). */
+  double Gg(double k, double m, double z, const Hod &h) const {
+    return ncen(h, m) + nsat(h, m) * unfw(k, m, z, h.f);
+  }
+  double Ggg(double k1, double k2, double m, double z, const Hod &h1, const Hod &h2,
+             bool same) const {
+    double Nc1 = ncen(h1, m);
+    double Nc2 = ncen(h2, m);
+    double Ns1 = nsat(h1, m);
+    double Ns2 = nsat(h2, m);
+    double Nss = nsatnsat(h1, h2, same, A, eps, m);
+    return Nc1 * Ns2 * unfw(k2, m, z, h2.f) + Nc2 * Ns1 * unfw(k1, m, z, h1.f) +
+           Nss * unfw(k1, m, z, h1.f) * unfw(k2, m, z, h2.f);
+  }
+
+  /* n_bar integrand (src/generalFunctions.cu:233-244). */
+  double k_nbar(double m, double z, const Hod &h) const {
+    double Nc = ncen(h, m);
+    double Ns = nsat(h, m);
+    int mi = mix_round(m);
+    int zi = zix_round(z);
+    return m * hmf[(size_t)(zi * nb + mi)] * (Nc + Ns);
+  }
+};
+
+/* Term context handed to the integrand callbacks. */
+struct TermCtx {
+  const Model *M;
+  int stat, term;
+  double t1, t2, t3;
+  const Hod *h1, *h2;
+  const double *nb1, *nb2;
+  bool same;
+  double z;  /* n_bar only */
+};
+
+/* NNMap 1-halo (src/NNMap_Model.cu:544-593). */
+double nnm_1h(const TermCtx &T, double l1, double l2, double phi, double m, double z) {
+  const Model &M = *T.M;
+  int zi = M.zix_round(z);
+  int mi = M.mix_round(m);
+  double nbar1 = T.nb1[zi], nbar2 = T.nb2[zi];
+  double w_ = M.w[zi];
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
+  double B = 1. / nbar1 / nbar2 / M.rho_bar[zi] * M.hmf[(size_t)zi * M.nb + mi] * m *
+             M.unfw(k3, m, z, 1);
+  B *= M.Ggg(k1, k2, m, z, *T.h1, *T.h2, T.same);
+  double b = B / M.dwdz[zi] * M.g[zi] * M.p1[zi] * M.p2[zi] / w_ / w_ / w_ * (1. + z);
+  return l1 * l2 * m * l1 * l2 * aperture(T.t1 * l1) * aperture(T.t2 * l2) *
+         aperture(T.t3 * l3) * b;
+}
+
+/* NNMap 2-halo (src/NNMap_Model.cu:787-856). */
+double nnm_2h(const TermCtx &T, double l1, double l2, double phi, double m1, double m2,
+              double z) {
+  const Model &M = *T.M;
+  const size_t nb = M.nb;
+  int zi = M.zix_round(z);
+  int i1 = M.mix_round(m1), i2 = M.mix_round(m2);
+  double nbar1 = T.nb1[zi], nbar2 = T.nb2[zi];
+  double w_ = M.w[zi];
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
+  double P1 = M.pk(zi, M.kix_round(k1), k1);
+  double P2 = M.pk(zi, M.kix_round(k2), k2);
+  double P3 = M.pk(zi, M.kix_round(k3), k3);
+  const Hod &h1 = *T.h1, &h2 = *T.h2;
+  double B = 1. / nbar1 / nbar2 / M.rho_bar[zi] * M.hmf[zi * nb + i1] * M.hmf[zi * nb + i2] *
+             M.bh[zi * nb + i1] * M.bh[zi * nb + i2] *
+             (m1 * M.unfw(k3, m1, z, 1) * M.Ggg(k1, k2, m2, z, h1, h2, T.same) * P3 +
+              m2 * M.unfw(k3, m2, z, 1) * M.Gg(k1, m1, z, h1) * M.Gg(k2, m2, z, h2) * P1 +
+              m2 * M.unfw(k3, m2, z, 1) * M.Gg(k2, m1, z, h2) * M.Gg(k1, m2, z, h1) * P2);
+  double b = 1. / M.dwdz[zi] * M.g[zi] * M.p1[zi] * M.p2[zi] / w_ / w_ / w_ * (1. + z) * B;
+  return l1 * l2 * m1 * m2 * l1 * l2 * aperture(T.t1 * l1) * aperture(T.t2 * l2) *
+         aperture(T.t3 * l3) * b;
+}
+
+/* NNMap 3-halo (src/NNMap_Model.cu:1003-1037). */
+double nnm_3h(const TermCtx &T, double l1, double l2, double phi, double m1, double m2,
+              double m3, double z) {
+  const Model &M = *T.M;
+  const size_t nb = M.nb;
+  int zi = M.zix_round(z);
+  int i1 = M.mix_round(m1), i2 = M.mix_round(m2), i3 = M.mix_round(m3);
+  double nbar1 = T.nb1[zi], nbar2 = T.nb2[zi];
+  double w_ = M.w[zi];
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
+  double B = 1. / nbar1 / nbar2 / M.rho_bar[zi] * M.hmf[zi * nb + i1] * M.hmf[zi * nb + i2] *
+             M.hmf[zi * nb + i3] * m3 * M.unfw(k3, m3, z, 1) * M.Gg(k1, m1, z, *T.h1) *
+             M.Gg(k2, m2, z, *T.h2) * M.bilin(k1, k2, std::cos(phi), z) * M.bh[zi * nb + i1] *
+             M.bh[zi * nb + i2] * M.bh[zi * nb + i3];
+  double b = B / M.dwdz[zi] * M.g[zi] * M.p1[zi] * M.p2[zi] / w_ / w_ / w_ * (1. + z);
+  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * aperture(T.t1 * l1) * aperture(T.t2 * l2) *
+         aperture(T.t3 * l3) * b;
+}
+
+/* NMapMap 1-halo (src/NMapMap_Model.cu:263-301). */
+double nmm_1h(const TermCtx &T, double l1, double l2, double phi, double m, double z) {
+  const Model &M = *T.M;
+  int zi = M.zix_round(z);
+  int mi = M.mix_round(m);
+  double nbar = T.nb1[zi];
+  double w_ = M.w[zi];
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
+  const Hod &h = *T.h1;
+  double Nc = ncen(h, m), Ns = nsat(h, m);
+  double rho = M.rho_bar[zi];
+  double B = 1. / nbar / rho / rho * M.hmf[(size_t)zi * M.nb + mi] * m * m *
+             (Nc + Ns * M.unfw(k1, m, z, h.f)) * M.unfw(k2, m, z, 1) * M.unfw(k3, m, z, 1);
+  double b = B / M.dwdz[zi] * M.g[zi] * M.g[zi] * M.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
+  return l1 * l2 * m * l1 * l2 * aperture(T.t1 * l1) * aperture(T.t2 * l2) *
+         aperture(T.t3 * l3) * b / nbar;
+}
+
+/* NMapMap 2-halo (src/NMapMap_Model.cu:402-468). */
+double nmm_2h(const TermCtx &T, double l1, double l2, double phi, double m1, double m2,
+              double z) {
+  const Model &M = *T.M;
+  const size_t nb = M.nb;
+  int zi = M.zix_round(z);
+  int i1 = M.mix_round(m1), i2 = M.mix_round(m2);
+  double nbar = T.nb1[zi];
+  double w_ = M.w[zi];
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
+  double P1 = M.pk(zi, M.kix_round(k1), k1);
+  double P2 = M.pk(zi, M.kix_round(k2), k2);
+  double P3 = M.pk(zi, M.kix_round(k3), k3);
+  const Hod &h = *T.h1;
+  double Nc1 = ncen(h, m1), Ns1 = nsat(h, m1);
+  double Nc2 = ncen(h, m2), Ns2 = nsat(h, m2);
+  double f = h.f;
+  double rho = M.rho_bar[zi];
+  double B = 1. / nbar / rho / rho * M.hmf[zi * nb + i1] * M.hmf[zi * nb + i2] *
+             M.bh[zi * nb + i1] * M.bh[zi * nb + i2] *
+             (m1 * m2 * M.unfw(k2, m1, z, 1) * M.unfw(k3, m2, z, 1) *
+                  (Nc2 + Ns2 * M.unfw(k1, m2, z, f)) * P2 +
+              m1 * m2 * M.unfw(k3, m1, z, 1) * M.unfw(k2, m2, z, 1) *
+                  (Nc2 + Ns2 * M.unfw(k1, m2, z, f)) * P3 +
+              m2 * m2 * M.unfw(k2, m1, z, 1) * M.unfw(k3, m2, z, 1) *
+                  (Nc1 + Ns1 * M.unfw(k1, m1, z, f)) * P1);
+  double b = B / M.dwdz[zi] * M.g[zi] * M.g[zi] * M.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
+  return l1 * l2 * m1 * m2 * l1 * l2 * aperture(T.t1 * l1) * aperture(T.t2 * l2) *
+         aperture(T.t3 * l3) * b / nbar;
+}
+
+/* NMapMap 3-halo (src/NMapMap_Model.cu:593-632). */
+double nmm_3h(const TermCtx &T, double l1, double l2, double phi, double m1, double m2,
+              double m3, double z) {
+  const Model &M = *T.M;
+  const size_t nb = M.nb;
+  int zi = M.zix_round(z);
+  int i1 = M.mix_round(m1), i2 = M.mix_round(m2), i3 = M.mix_round(m3);
+  double nbar = T.nb1[zi];
+  double w_ = M.w[zi];
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
+  const Hod &h = *T.h1;
+  double Nc = ncen(h, m3), Ns = nsat(h, m3);
+  double rho = M.rho_bar[zi];
+  double B = 1. / nbar / rho / rho * M.hmf[zi * nb + i1] * M.hmf[zi * nb + i2] *
+             M.hmf[zi * nb + i3] * M.bh[zi * nb + i1] * M.bh[zi * nb + i2] *
+             M.bh[zi * nb + i3] * m1 * m2 * M.unfw(k2, m1, z, 1) * M.unfw(k3, m2, z, 1) *
+             (Nc + Ns * M.unfw(k1, m2, z, h.f)) * M.bilin(k1, k2, std::cos(phi), z);
+  double b = B / M.dwdz[zi] * M.g[zi] * M.g[zi] * M.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
+  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * aperture(T.t1 * l1) * aperture(T.t2 * l2) *
+         aperture(T.t3 * l3) * b / nbar;
+}
+
+/* Point loop of the integrand callbacks (src/NNMap_Model.cu:517-538 and
+ * the analogous loops): coordinates are log10 l1, log10 l2, phi,
+ * log10 m..., z. */
+int term_cb(unsigned ndim, size_t npts, const double *x, void *fd, unsigned fdim, double *v) {
+  if (fdim != 1) return 1;
+  const TermCtx &T = *(const TermCtx *)fd;
+  const long n = (long)npts;
+  const int key = T.stat * 4 + T.term;
+#pragma omp parallel for
+  for (long i = 0; i < n; ++i) {
+    const double *p = x + (size_t)i * ndim;
+    double r = 0;
+    switch (key) {
+      case 0: { /* n_bar */
+        double m = std::pow(10, p[0]);
+        r = T.M->k_nbar(m, T.z, *T.h1);
+        break;
+      }
+      case 1:
+        r = nnm_1h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]), p[4]);
+        break;
+      case 2:
+        r = nnm_2h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]),
+                   std::pow(10, p[4]), p[5]);
+        break;
+      case 3:
+        r = nnm_3h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]),
+                   std::pow(10, p[4]), std::pow(10, p[5]), p[6]);
+        break;
+      case 5:
+        r = nmm_1h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]), p[4]);
+        break;
+      case 6:
+        r = nmm_2h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]),
+                   std::pow(10, p[4]), p[5]);
+        break;
+      case 7:
+        r = nmm_3h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]),
+                   std::pow(10, p[4]), std::pow(10, p[5]), p[6]);
+        break;
+      default:
+        r = 0;
+    }
+    v[i] = r;
+  }
+  return (key == 4 || key > 7) ? 1 : 0;
+}
+
+/* Select HOD and n_bar by type (src/NNMap_Model.cu:437-475); NMapMap has
+ * one HOD kept in slot 1. */
+void bind_types(const Model &M, TermCtx &T, int type1, int type2) {
+  if (T.stat == ORC_NMAPMAP) {
+    T.h1 = T.h2 = &M.hod1;
+    T.nb1 = T.nb2 = M.nbar1.data();
+    T.same = true;
+    return;
+  }
+  bool a1 = (type1 == 1), a2 = (type2 == 1);
+  T.h1 = a1 ? &M.hod1 : &M.hod2;
+  T.nb1 = a1 ? M.nbar1.data() : M.nbar2.data();
+  T.h2 = a2 ? &M.hod1 : &M.hod2;
+  T.nb2 = a2 ? M.nbar1.data() : M.nbar2.data();
+  T.same = (a1 == a2) || M.identical;
+}
+
+/* Term boxes (src/NNMap_Model.cu:313-314, 338-339, 363-364). */
+unsigned term_box(const Model &M, int term, double *lo, double *hi) {
+  const double lmin = 0.1, lmax = 1e6;
+  const double phimax = 2 * kPi;
+  unsigned d = 0;
+  lo[d] = std::log10(lmin); hi[d++] = std::log10(lmax);
+  lo[d] = std::log10(lmin); hi[d++] = std::log10(lmax);
+  lo[d] = 0; hi[d++] = phimax;
+  for (int i = 0; i < term; ++i) { lo[d] = std::log10(M.mmin_i); hi[d++] = std::log10(M.mmax_i); }
+  lo[d] = M.zmin_i; hi[d++] = M.zmax_i;
+  return d;
+}
+
+/* Prefactors (src/NNMap_Model.cu:321-322, 346-347, 371-372;
+ * src/NMapMap_Model.cu:188-190, 212-214, 235-237). */
+double prefactor(const Model &M, int stat, int term, double r) {
+  const double tp = 2 * kPi;
+  if (stat == ORC_NNMAP) {
+    double v = r * 3 * M.H0 * M.H0 * M.Om / 2 / kC / kC;
+    for (int i = 0; i < 2 + term; ++i) v = v * kLn10;
+    return v / tp / tp / tp;
+  }
+  double v = r * 9 * std::pow(M.H0, 4) * std::pow(M.Om, 2) / 4 / std::pow(kC, 4);
+  for (int i = 0; i < 2 + term; ++i) v = v * kLn10;
+  return v / tp / tp / tp;
+}
+
+/* rho_bar and n_bar per z bin (src/NNMap_Model.cu:244-297). */
+int update_densities(Model &M, orc_hcubature_v_fn hcub) {
+  double lo[1] = {std::log10(M.mmin_i)}, hi[1] = {std::log10(M.mmax_i)};
+  M.rho_bar.assign(M.nb, 0);
+  M.nbar1.assign(M.nb, 0);
+  M.nbar2.assign(M.nb, 0);
+  for (int i = 0; i < M.nb; ++i) {
+    double z = M.zmin + i * (M.zmax - M.zmin) / M.nb;
+    double E = M.Om * (1 + z) * (1 + z) * (1 + z) + M.OL;
+    double Omz = M.Om * (1 + z) * (1 + z) * (1 + z) / E;
+    M.rho_bar[i] = Omz * M.rho_crit;
+  }
+  for (int i = 0; i < M.nb; ++i) {
+    double z = M.zmin + i * (M.zmax - M.zmin) / M.nb;
+    for (int which = 0; which < 2; ++which) {
+      TermCtx T{};
+      T.M = &M;
+      T.stat = 0;
+      T.term = 0;
+      T.z = z;
+      T.h1 = T.h2 = which == 0 ? &M.hod1 : &M.hod2;
+      double val = 0, err = 0;
+      int rc = run_cubature(hcub, term_cb, &T, 1, lo, hi, 0, 1e-4, &val, &err, nullptr,
+                            nullptr, 0);
+      if (rc) return rc;
+      (which == 0 ? M.nbar1 : M.nbar2)[i] = kLn10 * val;
+    }
+  }
+  return 0;
+}
+
+/* Genz (1984) test families on a box. */
+struct GenzCtx {
+  int family;
+  unsigned dim;
+  double a[kMaxDim], u[kMaxDim];
+};
+
+int genz_cb(unsigned ndim, size_t npts, const double *x, void *fd, unsigned, double *v) {
+  const GenzCtx &G = *(const GenzCtx *)fd;
+  for (size_t i = 0; i < npts; ++i) {
+    const double *p = x + i * ndim;
+    double r = 0;
+    switch (G.family) {
+      case 0: { /* oscillatory */
+        double s = 2 * M_PI * G.u[0];
+        for (unsigned j = 0; j < ndim; ++j) s += G.a[j] * p[j];
+        r = std::cos(s);
+        break;
+      }
+      case 1: { /* product peak */
+        r = 1;
+        for (unsigned j = 0; j < ndim; ++j) {
+          double t = p[j] - G.u[j];
+          r /= (1.0 / (G.a[j] * G.a[j]) + t * t);
+        }
+        break;
+      }
+      case 2: { /* corner peak */
+        double s = 1;
+        for (unsigned j = 0; j < ndim; ++j) s += G.a[j] * p[j];
+        r = std::pow(s, -(double)(ndim + 1));
+        break;
+      }
+      case 3: { /* Gaussian */
+        double s = 0;
+        for (unsigned j = 0; j < ndim; ++j) {
+          double t = p[j] - G.u[j];
+          s += G.a[j] * G.a[j] * t * t;
+        }
+        r = std::exp(-s);
+        break;
+      }
+      case 4: { /* C0 */
+        double s = 0;
+        for (unsigned j = 0; j < ndim; ++j) s += G.a[j] * std::fabs(p[j] - G.u[j]);
+        r = std::exp(-s);
+        break;
+      }
+      default: { /* monomial prod x_j^a_j (a as integer exponents) */
+        r = 1;
+        for (unsigned j = 0; j < ndim; ++j) r *= std::pow(p[j], G.a[j]);
+      }
+    }
+    v[i] = r;
+  }
+  return 0;
+}
+
+}  // namespace
+
+/* ---------------------------------------------------------------------- */
+/* C API                                                                  */
+/* ---------------------------------------------------------------------- */
+
+extern "C" {
+
+int orc_hcubature_v(unsigned fdim, orc_integrand_v f, void *fdata, unsigned dim,
+                    const double *xmin, const double *xmax, size_t maxEval, double reqAbsError,
+                    double reqRelError, int norm, double *val, double *err) {
+  (void)norm;
+  if (fdim != 1) return 1; /* the path only integrates scalar integrands */
+  return cubature(f, fdata, dim, xmin, xmax, maxEval, reqAbsError, reqRelError, val, err);
+}
+
+void *orc_model_create(int nbins, double zmin, double zmax, double kmin, double kmax,
+                       double mmin, double mmax, const double *g, const double *p_lens1,
+                       const double *p_lens2, const double *w, const double *dwdz,
+                       const double *hmf, const double *P_lin, const double *b_h,
+                       const double *concentration, double H0, double Omega_m,
+                       double Omega_Lambda) {
+  if (nbins < 2) return nullptr;
+  Model *M = new Model();
+  M->nb = nbins;
+  M->zmin = zmin; M->zmax = zmax; M->kmin = kmin; M->kmax = kmax;
+  M->mmin = mmin; M->mmax = mmax;
+  size_t n1 = nbins, n2 = (size_t)nbins * nbins;
+  M->g.assign(g, g + n1);
+  M->p1.assign(p_lens1, p_lens1 + n1);
+  M->p2.assign(p_lens2, p_lens2 + n1);
+  M->w.assign(w, w + n1);
+  M->dwdz.assign(dwdz, dwdz + n1);
+  M->hmf.assign(hmf, hmf + n2);
+  M->plin.assign(P_lin, P_lin + n2);
+  M->plin.resize(n2 + n1, 0.0);
+  M->bh.assign(b_h, b_h + n2);
+  M->conc.assign(concentration, concentration + n2);
+  M->H0 = H0; M->Om = Omega_m; M->OL = Omega_Lambda;
+  M->rho_crit = 3 * H0 * H0 * kMpcKm / 8 / kPi / kGMsun;
+  /* Integral borders (src/NNMap_Model.cu:46-49). */
+  M->zmin_i = zmin;
+  M->zmax_i = zmax - (zmax - zmin) / nbins;
+  M->mmin_i = mmin;
+  M->mmax_i = std::exp(std::log(mmin) + std::log(mmax / mmin) * (nbins - 1) / nbins);
+  return M;
+}
+
+void orc_model_destroy(void *model) { delete (Model *)model; }
+
+int orc_model_set_hods(void *model, const double *hod1, const double *hod2, double A,
+                       double epsilon, int identical, orc_hcubature_v_fn hcub) {
+  Model &M = *(Model *)model;
+  M.hod1 = hod_from(hod1);
+  M.hod2 = hod_from(hod2 ? hod2 : hod1);
+  M.identical = identical != 0;
+  M.A = A;
+  M.eps = epsilon;
+  return update_densities(M, hcub);
+}
+
+int orc_model_densities(void *model, double *rho_bar, double *n_bar1, double *n_bar2) {
+  const Model &M = *(const Model *)model;
+  if ((int)M.rho_bar.size() != M.nb) return 1;
+  for (int i = 0; i < M.nb; ++i) {
+    if (rho_bar) rho_bar[i] = M.rho_bar[i];
+    if (n_bar1) n_bar1[i] = M.nbar1[i];
+    if (n_bar2) n_bar2[i] = M.nbar2[i];
+  }
+  return 0;
+}
+
+int orc_model_borders(void *model, double *out4) {
+  const Model &M = *(const Model *)model;
+  out4[0] = M.zmin_i; out4[1] = M.zmax_i; out4[2] = M.mmin_i; out4[3] = M.mmax_i;
+  return 0;
+}
+
+int orc_eval_term(void *model, int stat, int term, double theta1, double theta2,
+                  double theta3, int type1, int type2, double reltol, uint64_t maxeval,
+                  orc_hcubature_v_fn hcub, double *out, orc_stats *stats, uint32_t *trace,
+                  uint32_t trace_cap) {
+  const Model &M = *(const Model *)model;
+  if (term < 1 || term > 3 || (stat != ORC_NNMAP && stat != ORC_NMAPMAP)) return 1;
+  if ((int)M.rho_bar.size() != M.nb) return 2;
+  TermCtx T{};
+  T.M = &M;
+  T.stat = stat;
+  T.term = term;
+  T.t1 = theta1; T.t2 = theta2; T.t3 = theta3;
+  bind_types(M, T, type1, type2);
+  double lo[8], hi[8];
+  unsigned dim = term_box(M, term, lo, hi);
+  double val = 0, err = 0;
+  int rc = run_cubature(hcub, term_cb, &T, dim, lo, hi, (size_t)maxeval, reltol, &val, &err,
+                        stats, trace, trace_cap);
+  out[0] = prefactor(M, stat, term, val);
+  out[1] = val;
+  out[2] = err;
+  return rc;
+}
+
+int orc_integrand(void *model, int stat, int term, double theta1, double theta2,
+                  double theta3, int type1, int type2, size_t npts, const double *x,
+                  double *f) {
+  const Model &M = *(const Model *)model;
+  TermCtx T{};
+  T.M = &M;
+  T.stat = stat;
+  T.term = term;
+  T.t1 = theta1; T.t2 = theta2; T.t3 = theta3;
+  if (term == 0) {
+    T.stat = 0;
+    T.z = theta1;
+    T.h1 = T.h2 = (type1 == 2) ? &M.hod2 : &M.hod1;
+    return term_cb(1, npts, x, &T, 1, f);
+  }
+  bind_types(M, T, type1, type2);
+  return term_cb(4 + term, npts, x, &T, 1, f);
+}
+
+int orc_genz(int family, unsigned dim, const double *a, const double *u, const double *xmin,
+             const double *xmax, uint64_t maxeval, double reltol, orc_hcubature_v_fn hcub,
+             double *val, double *err, orc_stats *stats) {
+  if (dim == 0 || dim > kMaxDim) return 1;
+  GenzCtx G{};
+  G.family = family;
+  G.dim = dim;
+  for (unsigned i = 0; i < dim; ++i) { G.a[i] = a[i]; G.u[i] = u[i]; }
+  return run_cubature(hcub, genz_cb, &G, dim, xmin, xmax, (size_t)maxeval, reltol, val, err,
+                      stats, nullptr, 0);
+}
+
+void orc_sici(double x, double *si, double *ci) { sici(x, *si, *ci); }
+double orc_aperture(double eta) { return aperture(eta); }
+double orc_ncen(const double *hod, double m) { return ncen(hod_from(hod), m); }
+double orc_nsat(const double *hod, double m) { return nsat(hod_from(hod), m); }
+double orc_nsatnsat(const double *hod1, const double *hod2, int same, double A, double epsilon,
+                    double m) {
+  return nsatnsat(hod_from(hod1), hod_from(hod2), same != 0, A, epsilon, m);
+}
+double orc_unfw(void *model, double k, double m, double z, double f) {
+  return ((const Model *)model)->unfw(k, m, z, f);
+}
+double orc_bilin(void *model, double k1, double k2, double cosphi, double z) {
+  return ((const Model *)model)->bilin(k1, k2, cosphi, z);
+}
+double orc_fkernel(double k1, double k2, double cosphi) { return fkernel(k1, k2, cosphi); }
+
+int orc_rule_one_region(unsigned dim, orc_integrand_v f, void *fdata, const double *center,
+                        const double *halfwidth, double *val, double *err, unsigned *split) {
+  if (dim == 0 || dim > kMaxDim) return 1;
+  std::vector<Region> R(1);
+  for (unsigned i = 0; i < dim; ++i) { R[0].c[i] = center[i]; R[0].h[i] = halfwidth[i]; }
+  R[0].vol = box_volume(dim, R[0].h);
+  std::vector<double> pts, vals;
+  if (eval_batch(dim, R, 1, f, fdata, pts, vals)) return 1;
+  *val = R[0].val;
+  *err = R[0].err;
+  *split = R[0].split;
+  return 0;
+}
+
+uint32_t orc_points_per_region(unsigned dim) { return points_per_region(dim); }
+
+}  // extern "C"
