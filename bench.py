@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark: NNMap adaptive integrals per second on one MI355X.
+
+Workload (default ``nnmap550``): NNMap 1h + 2h + 3h for the 550 aperture
+triplets (10 log-spaced apertures 0.5-32 arcmin, theta1 <= theta2, 10 theta3),
+fiducial HOD, relative tolerance 1e-4 for every term, 5e6 evaluations per
+integral at most -- 1650 adaptive integrals of 5-7 dimensions.  One step is
+one pass of the batched device-resident engine (g3l_eval_batch, FP64) over
+all 1650 integrals; the tables are resident in HBM before timing starts.
+
+Beside the batched value the line reports the plain one-integral-at-a-time
+HIP path (g3l_eval_plain) on a fixed sample, the FP64 VALU roofline of the
+evaluation kernel and the CPU oracle (FP64 C++/OpenMP restatement of the
+reference CPU build) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from g3lhalo_amd import _lib, flops  # noqa: E402
+from g3lhalo_amd import tables as T  # noqa: E402
+
+FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+
+
+def workload(name: str, hodset: int = 0):
+    reqs = []
+    if name == "nnmap550":
+        for th in T.triplets_550():
+            for term in (1, 2, 3):
+                reqs.append(_lib.make_request(0, term, th * T.ARCMIN_IN_RAD, hodset=hodset,
+                                              type1=1, type2=2, reltol=1e-4, maxeval=5000000))
+        desc = {"workload": "NNMap 1h+2h+3h, 550 triplets 0.5-32 arcmin, tol 1e-4, maxEval 5e6",
+                "integrals": len(reqs), "nbins": 128, "hod": "fiducial", "types": "(1,2)"}
+    elif name == "nnmap4":
+        for th in T.triplets_equal_4():
+            reqs.append(_lib.make_request(0, 1, th * T.ARCMIN_IN_RAD, hodset=hodset, type1=1,
+                                          type2=2, reltol=1e-3, maxeval=5000000))
+        desc = {"workload": "NNMap 1h, 4 equal-aperture triplets 1-8 arcmin, tol 1e-3",
+                "integrals": len(reqs), "nbins": 128}
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return reqs, desc
+
+
+def algorithmic_flops(reqs, results) -> float:
+    return float(sum(int(r.neval) * flops.flops_per_eval(q.stat, q.term)
+                     for q, r in zip(reqs, results)))
+
+
+def cpu_baseline(tables, reqs, seconds_budget: float):
+    """Oracle (FP64 C++/OpenMP port of the reference CPU build) on a sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    m = O.Model(tables)
+    m.set_hods(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
+    idx = np.linspace(0, len(reqs) - 1, 12).astype(int)
+    done, t_used = 0, 0.0
+    vals = {}
+    for i in idx:
+        q = reqs[i]
+        t0 = time.perf_counter()
+        o = m.eval_term(q.stat, q.term, list(q.theta), q.type1, q.type2, q.reltol, q.maxeval)
+        t_used += time.perf_counter() - t0
+        vals[int(i)] = o["value"]
+        done += 1
+        if t_used > seconds_budget:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": done / t_used, "unit": "integrals/s", "cores": threads, "kind": "port",
+            "sample": f"{done} integrals of the workload (indices {list(vals)}), full maxEval",
+            "seconds": round(t_used, 2)}, vals
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="nnmap550")
+    ap.add_argument("--plain-sample", type=int, default=6)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the integrals are independent and shard by request "
+              "with no exchange; run it with --gpus 1", file=sys.stderr)
+        return 2
+
+    tables = T.make_tables()
+    ctx = _lib.Context(tables)
+    sid = ctx.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
+    reqs, desc = workload(a.workload, sid)
+
+    for _ in range(a.warmup):
+        ctx.eval_batch(reqs)
+    step_ms, eval_ms, flop_total = [], 0.0, 0.0
+    results = None
+    launches = 0
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        results = ctx.eval_batch(reqs)
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+        st = ctx.last_stats()
+        eval_ms += st.eval_ms
+        launches += st.eval_launches
+        flop_total += algorithmic_flops(reqs, results)
+    elapsed = time.perf_counter() - t_start
+    value = len(reqs) * a.steps / elapsed
+    bad = [r.status for r in results if r.status not in (_lib.G3L_OK, _lib.G3L_MAXEVAL_HIT)]
+
+    # plain one-integral-at-a-time HIP path on a fixed sample
+    pidx = np.linspace(0, len(reqs) - 1, max(1, a.plain_sample)).astype(int)
+    t0 = time.perf_counter()
+    plain = {int(i): ctx.eval_plain(reqs[i]) for i in pidx}
+    plain_s = time.perf_counter() - t0
+    plain_rate = len(pidx) / plain_s
+    dev = max(abs(plain[i].value - results[i].value) / max(abs(plain[i].value), 1e-300)
+              for i in pidx)
+
+    achieved = flop_total / (eval_ms * 1e-3) / 1e12 if eval_ms > 0 else 0.0
+    line = {
+        "metric": "NNMap integrals/sec at 1e-4 rel tol (batched vs plain, 1 GPU); p50/p99 latency",
+        "value": round(value, 3),
+        "unit": "integrals/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(float(np.mean(step_ms)), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic tables (EH no-wiggle P(k), Sheth-Tormen, NFW c(m,z)) on a 128-bin grid",
+        "config": dict(desc, form="batched device-resident, FP64"),
+        "plain": {"value": round(plain_rate, 3), "unit": "integrals/s",
+                  "sample": f"{len(pidx)} integrals of the workload",
+                  "speedup_batched_over_plain": round(value / plain_rate, 2),
+                  "max_rel_diff_batched_vs_plain": float(dev)},
+        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
+                       "p99": round(float(np.percentile(step_ms, 99)), 3),
+                       "note": "batched form: every integral of a step completes with the step"},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "k_eval (FP64 fused point generation + integrand + rule)",
+                     "eval_ms_total": round(eval_ms, 3), "eval_launches": int(launches)},
+        "evals_per_step": int(sum(int(r.neval) for r in results)),
+        "failed_integrals": len(bad),
+    }
+    if not a.no_cpu:
+        cb, _ = cpu_baseline(tables, reqs, a.cpu_seconds)
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
