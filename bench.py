@@ -30,6 +30,8 @@ from g3lhalo_amd import _lib, flops  # noqa: E402
 from g3lhalo_amd import tables as T  # noqa: E402
 
 FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapMap 1h",
+              5: "NMapMap 2h", 6: "NMapMap 3h"}
 
 
 def workload(name: str, hodset: int = 0):
@@ -50,11 +52,6 @@ def workload(name: str, hodset: int = 0):
     else:
         raise SystemExit(f"unknown workload {name}")
     return reqs, desc
-
-
-def algorithmic_flops(reqs, results) -> float:
-    return float(sum(int(r.neval) * flops.flops_per_eval(q.stat, q.term)
-                     for q, r in zip(reqs, results)))
 
 
 def cpu_baseline(tables, reqs, seconds_budget: float):
@@ -104,9 +101,11 @@ def main(argv=None):
 
     for _ in range(a.warmup):
         ctx.eval_batch(reqs)
-    step_ms, eval_ms, flop_total = [], 0.0, 0.0
+    step_ms, eval_ms = [], 0.0
+    kind_ms = np.zeros(8)
+    kind_launches = np.zeros(8, dtype=np.int64)
+    kind_evals = np.zeros(8, dtype=np.int64)
     results = None
-    launches = 0
     t_start = time.perf_counter()
     for _ in range(a.steps):
         t0 = time.perf_counter()
@@ -114,8 +113,9 @@ def main(argv=None):
         step_ms.append((time.perf_counter() - t0) * 1e3)
         st = ctx.last_stats()
         eval_ms += st.eval_ms
-        launches += st.eval_launches
-        flop_total += algorithmic_flops(reqs, results)
+        kind_ms += np.array(st.kind_ms[:])
+        kind_launches += np.array(st.kind_launches[:], dtype=np.int64)
+        kind_evals += np.array(st.kind_evals[:], dtype=np.int64)
     elapsed = time.perf_counter() - t_start
     value = len(reqs) * a.steps / elapsed
     bad = [r.status for r in results if r.status not in (_lib.G3L_OK, _lib.G3L_MAXEVAL_HIT)]
@@ -129,7 +129,21 @@ def main(argv=None):
     dev = max(abs(plain[i].value - results[i].value) / max(abs(plain[i].value), 1e-300)
               for i in pidx)
 
-    achieved = flop_total / (eval_ms * 1e-3) / 1e12 if eval_ms > 0 else 0.0
+    # roofline of the dominant evaluation kernel (largest share of device time):
+    # algorithmic flops = its evaluations x frozen flops per evaluation
+    per_kind = {}
+    for k in range(7):
+        if kind_launches[k] == 0:
+            continue
+        fl = float(kind_evals[k]) * flops.FLOPS_PER_EVAL[k]
+        per_kind[KIND_NAMES[k]] = {
+            "ms_total": round(float(kind_ms[k]), 3), "launches": int(kind_launches[k]),
+            "avg_launch_ms": round(float(kind_ms[k] / kind_launches[k]), 4),
+            "evals": int(kind_evals[k]), "flops_per_eval": flops.FLOPS_PER_EVAL[k],
+            "tflops": round(fl / (kind_ms[k] * 1e-3) / 1e12, 3)}
+    dom = int(np.argmax(kind_ms[:7]))
+    dom_flops = float(kind_evals[dom]) * flops.FLOPS_PER_EVAL[dom]
+    achieved = dom_flops / (kind_ms[dom] * 1e-3) / 1e12 if kind_ms[dom] > 0 else 0.0
     line = {
         "metric": "NNMap integrals/sec at 1e-4 rel tol (batched vs plain, 1 GPU); p50/p99 latency",
         "value": round(value, 3),
@@ -154,8 +168,10 @@ def main(argv=None):
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_eval (FP64 fused point generation + integrand + rule)",
-                     "eval_ms_total": round(eval_ms, 3), "eval_launches": int(launches)},
+                     "kernel": f"k_eval<{dom}> ({KIND_NAMES[dom]}, FP64 fused points + "
+                               "integrand + rule)",
+                     "avg_launch_ms": round(float(kind_ms[dom] / max(kind_launches[dom], 1)), 4),
+                     "per_kernel": per_kind, "eval_ms_total": round(eval_ms, 3)},
         "evals_per_step": int(sum(int(r.neval) for r in results)),
         "failed_integrals": len(bad),
     }

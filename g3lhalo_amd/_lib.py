@@ -51,7 +51,8 @@ class Result(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("eval_ms", C.c_double), ("select_ms", C.c_double), ("wall_ms", C.c_double),
                 ("eval_launches", C.c_uint64), ("evals", C.c_uint64), ("rounds", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("pad", C.c_uint32), ("kind_ms", C.c_double * 8),
+                ("kind_launches", C.c_uint64 * 8), ("kind_evals", C.c_uint64 * 8)]
 
 
 # Every symbol include/g3l.h declares, with its ctypes signature.

@@ -121,6 +121,7 @@ struct g3l_ctx {
   HostBuf hgeo, hsplit;
   uint32_t pcap = 0;
   hipEvent_t ev[4]{};
+  hipEvent_t kev[2 * kNumKinds]{};
   g3l_batch_stats last{};
 };
 
@@ -299,6 +300,9 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
   SL.counters = dcnt;
 
   double eval_ms = 0, select_ms = 0;
+  double kind_ms[kNumKinds] = {0};
+  uint64_t kind_launches[kNumKinds] = {0};
+  bool kind_used[kNumKinds];
   uint64_t launches = 0;
   uint32_t rounds = 0;
   G3L_HIP(hipMemcpyAsync(hcnt, dcnt, (kNumKinds + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -306,11 +310,15 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
   for (;;) {
     if (timed) G3L_HIP(hipEventRecord(c->ev[0], s));
     for (int k = 0; k < kNumKinds; ++k) {
-      if (hcnt[k] == 0) continue;
+      kind_used[k] = hcnt[k] != 0;
+      if (!kind_used[k]) continue;
       EL.list = lists[k];
       EL.count = (uint32_t)std::min<uint64_t>(hcnt[k], per_kind[k]);
+      if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k], s));
       G3L_HIP(launch_eval(k, EL, s));
+      if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k + 1], s));
       ++launches;
+      ++kind_launches[k];
     }
     if (timed) G3L_HIP(hipEventRecord(c->ev[1], s));
     G3L_HIP(launch_zero(dcnt, kNumKinds + 1, s));
@@ -325,6 +333,12 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
       G3L_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
       eval_ms += a;
       select_ms += b;
+      for (int k = 0; k < kNumKinds; ++k) {
+        if (!kind_used[k]) continue;
+        float m = 0;
+        G3L_HIP(hipEventElapsedTime(&m, c->kev[2 * k], c->kev[2 * k + 1]));
+        kind_ms[k] += m;
+      }
     }
     ++rounds;
     if (hcnt[kNumKinds] == 0) break;
@@ -332,7 +346,11 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
   G3L_HIP(hipMemcpyAsync(out.data(), c->state.p, nj * sizeof(JobState), hipMemcpyDeviceToHost, s));
   G3L_HIP(hipStreamSynchronize(s));
   uint64_t evals = 0;
-  for (auto &st : out) evals += st.neval;
+  uint64_t kind_evals[kNumKinds] = {0};
+  for (uint32_t j = 0; j < nj; ++j) {
+    evals += out[j].neval;
+    kind_evals[E.desc[j].kind] += out[j].neval;
+  }
   auto t1 = std::chrono::steady_clock::now();
   c->last.eval_ms = eval_ms;
   c->last.select_ms = select_ms;
@@ -340,6 +358,11 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
   c->last.eval_launches = launches;
   c->last.evals = evals;
   c->last.rounds = rounds;
+  for (int k = 0; k < 8; ++k) {
+    c->last.kind_ms[k] = k < kNumKinds ? kind_ms[k] : 0;
+    c->last.kind_launches[k] = k < kNumKinds ? kind_launches[k] : 0;
+    c->last.kind_evals[k] = k < kNumKinds ? kind_evals[k] : 0;
+  }
   return G3L_OK;
 }
 
@@ -630,6 +653,7 @@ int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
     return G3L_EHIP;
   }
   for (auto &e : c->ev) hipEventCreate(&e);
+  for (auto &e : c->kev) hipEventCreate(&e);
   const size_t n1 = nb, n2 = (size_t)nb * nb;
   const size_t total = 6 * n1 + 4 * n2 + n1;
   if (c->tables.ensure(total * sizeof(double))) {
@@ -679,6 +703,8 @@ int g3l_ctx_destroy(g3l_ctx *c) {
   c->pgeo.release(); c->psplit.release(); c->plist.release(); c->pjob.release();
   c->parg.release(); c->hgeo.release(); c->hsplit.release();
   for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
+  for (auto &e : c->kev)
     if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;

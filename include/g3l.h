@@ -140,6 +140,10 @@ typedef struct g3l_batch_stats {
   uint64_t evals;        /* integrand evaluations */
   uint32_t rounds;
   uint32_t pad;
+  /* per integrand kind (0 n_bar, 1-3 NNMap 1h-3h, 4-6 NMapMap 1h-3h) */
+  double kind_ms[8];
+  uint64_t kind_launches[8];
+  uint64_t kind_evals[8];
 } g3l_batch_stats;
 int g3l_last_batch_stats(g3l_ctx *ctx, g3l_batch_stats *out);
 
