@@ -88,6 +88,7 @@ def main(argv=None):
     ap.add_argument("--plain-sample", type=int, default=6)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the integrals are independent and shard by request "
@@ -99,26 +100,35 @@ def main(argv=None):
     sid = ctx.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
     reqs, desc = workload(a.workload, sid)
 
-    for _ in range(a.warmup):
-        ctx.eval_batch(reqs)
-    step_ms, eval_ms = [], 0.0
-    kind_ms = np.zeros(8)
-    kind_launches = np.zeros(8, dtype=np.int64)
-    kind_evals = np.zeros(8, dtype=np.int64)
-    results = None
-    t_start = time.perf_counter()
-    for _ in range(a.steps):
-        t0 = time.perf_counter()
-        results = ctx.eval_batch(reqs)
-        step_ms.append((time.perf_counter() - t0) * 1e3)
-        st = ctx.last_stats()
-        eval_ms += st.eval_ms
-        kind_ms += np.array(st.kind_ms[:])
-        kind_launches += np.array(st.kind_launches[:], dtype=np.int64)
-        kind_evals += np.array(st.kind_evals[:], dtype=np.int64)
-    elapsed = time.perf_counter() - t_start
-    value = len(reqs) * a.steps / elapsed
+    def run_form(precision):
+        for _ in range(a.warmup):
+            ctx.eval_batch(reqs, precision=precision)
+        step_ms = []
+        kind_ms = np.zeros(8)
+        kind_launches = np.zeros(8, dtype=np.int64)
+        kind_evals = np.zeros(8, dtype=np.int64)
+        res = None
+        t_start = time.perf_counter()
+        for _ in range(a.steps):
+            t0 = time.perf_counter()
+            res = ctx.eval_batch(reqs, precision=precision)
+            step_ms.append((time.perf_counter() - t0) * 1e3)
+            st = ctx.last_stats()
+            kind_ms += np.array(st.kind_ms[:])
+            kind_launches += np.array(st.kind_launches[:], dtype=np.int64)
+            kind_evals += np.array(st.kind_evals[:], dtype=np.int64)
+        elapsed = time.perf_counter() - t_start
+        return {"value": len(reqs) * a.steps / elapsed, "step_ms": step_ms, "results": res,
+                "kind_ms": kind_ms, "kind_launches": kind_launches, "kind_evals": kind_evals}
+
+    f64 = run_form(_lib.FP64)
+    mixed = run_form(_lib.MIXED) if not a.fp64_only else None
+    head = mixed if mixed is not None else f64
+    value = head["value"]
+    step_ms = head["step_ms"]
+    results = f64["results"]
     bad = [r.status for r in results if r.status not in (_lib.G3L_OK, _lib.G3L_MAXEVAL_HIT)]
+    kind_ms, kind_launches, kind_evals = head["kind_ms"], head["kind_launches"], head["kind_evals"]
 
     # plain one-integral-at-a-time HIP path on a fixed sample
     pidx = np.linspace(0, len(reqs) - 1, max(1, a.plain_sample)).astype(int)
@@ -128,6 +138,8 @@ def main(argv=None):
     plain_rate = len(pidx) / plain_s
     dev = max(abs(plain[i].value - results[i].value) / max(abs(plain[i].value), 1e-300)
               for i in pidx)
+    dev_head = max(abs(plain[i].value - head["results"][i].value) /
+                   max(abs(plain[i].value), 1e-300) for i in pidx)
 
     # roofline of the dominant evaluation kernel (largest share of device time):
     # algorithmic flops = its evaluations x frozen flops per evaluation
@@ -144,6 +156,12 @@ def main(argv=None):
     dom = int(np.argmax(kind_ms[:7]))
     dom_flops = float(kind_evals[dom]) * flops.FLOPS_PER_EVAL[dom]
     achieved = dom_flops / (kind_ms[dom] * 1e-3) / 1e12 if kind_ms[dom] > 0 else 0.0
+    peak = flops.mixed_peak_tflops(dom) if mixed is not None else FP64_VALU_PEAK_TFLOPS
+    mixed_dev = None
+    if mixed is not None:
+        mixed_dev = max(abs(m.value - r.value) / max(abs(r.value), 1e-300)
+                        for m, r in zip(mixed["results"], f64["results"]))
+
     line = {
         "metric": "NNMap integrals/sec at 1e-4 rel tol (batched vs plain, 1 GPU); p50/p99 latency",
         "value": round(value, 3),
@@ -155,23 +173,30 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed is not None else "f64",
         "data": "synthetic tables (EH no-wiggle P(k), Sheth-Tormen, NFW c(m,z)) on a 128-bin grid",
-        "config": dict(desc, form="batched device-resident, FP64"),
+        "config": dict(desc, form=("batched device-resident, mixed precision (FP32 "
+                                   "transcendental bulk, FP64 indices/masses/sums)"
+                                   if mixed is not None else "batched device-resident, FP64")),
+        "batched_fp64": {"value": round(f64["value"], 3), "unit": "integrals/s",
+                         "ms_per_step": round(float(np.mean(f64["step_ms"])), 3)},
+        "mixed_max_rel_vs_fp64_batched": mixed_dev,
         "plain": {"value": round(plain_rate, 3), "unit": "integrals/s",
                   "sample": f"{len(pidx)} integrals of the workload",
                   "speedup_batched_over_plain": round(value / plain_rate, 2),
-                  "max_rel_diff_batched_vs_plain": float(dev)},
+                  "speedup_batched_fp64_over_plain": round(f64["value"] / plain_rate, 2),
+                  "max_rel_diff_batched_fp64_vs_plain": float(dev),
+                  "max_rel_diff_batched_vs_plain": float(dev_head)},
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
                        "p99": round(float(np.percentile(step_ms, 99)), 3),
                        "note": "batched form: every integral of a step completes with the step"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
-                     "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": f"k_eval<{dom}> ({KIND_NAMES[dom]}, FP64 fused points + "
-                               "integrand + rule)",
+                     "peak": round(peak, 2), "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": f"k_eval<{dom},{'PMix' if mixed is not None else 'P64'}> "
+                               f"({KIND_NAMES[dom]}, fused points + integrand + rule)",
                      "avg_launch_ms": round(float(kind_ms[dom] / max(kind_launches[dom], 1)), 4),
-                     "per_kernel": per_kind, "eval_ms_total": round(eval_ms, 3)},
+                     "per_kernel": per_kind},
         "evals_per_step": int(sum(int(r.neval) for r in results)),
         "failed_integrals": len(bad),
     }

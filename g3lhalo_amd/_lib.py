@@ -70,7 +70,8 @@ SIGNATURES = {
     "g3l_eval_plain": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(Result)]),
     "g3l_eval_batch": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_size_t, C.c_int32,
                                  C.POINTER(Result)]),
-    "g3l_integrand": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_size_t, _dp, _dp]),
+    "g3l_integrand": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int32, C.c_size_t, _dp,
+                                _dp]),
     "g3l_last_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
 }
 
@@ -195,11 +196,11 @@ class Context:
         check("g3l_eval_batch", load().g3l_eval_batch(self.h, arr, n, int(precision), out))
         return list(out)
 
-    def integrand(self, req: Request, x):
+    def integrand(self, req: Request, x, precision=FP64):
         x = np.ascontiguousarray(x, dtype=np.float64)
         f = np.zeros(x.shape[0])
-        check("g3l_integrand", load().g3l_integrand(self.h, C.byref(req), x.shape[0], _p(x),
-                                                     _p(f)))
+        check("g3l_integrand", load().g3l_integrand(self.h, C.byref(req), int(precision),
+                                                     x.shape[0], _p(x), _p(f)))
         return f
 
     def last_stats(self) -> BatchStats:

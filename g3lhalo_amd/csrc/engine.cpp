@@ -145,6 +145,7 @@ TermArgs make_args(const g3l_ctx *c, const g3l_request &r) {
   a.t3 = r.theta[2];
   a.A = s.A;
   a.eps = s.eps;
+  a.lmmin = std::log(c->mmin);
   if (r.stat == G3L_NMAPMAP) {
     a.h1 = a.h2 = to_dev(s.h1);
     a.nb1 = a.nb2 = nbar_ptr(c, r.hodset, 0);
@@ -157,6 +158,10 @@ TermArgs make_args(const g3l_ctx *c, const g3l_request &r) {
     a.nb2 = nbar_ptr(c, r.hodset, a2 ? 0 : 1);
     a.same = (a1 == a2) || s.identical;
   }
+  a.lth1 = std::log(c->mmin / a.h1.mth);
+  a.lmp1 = std::log(c->mmin / a.h1.mprime);
+  a.lth2 = std::log(c->mmin / a.h2.mth);
+  a.lmp2 = std::log(c->mmin / a.h2.mprime);
   return a;
 }
 
@@ -223,7 +228,7 @@ struct EngineJobs {
   std::vector<double> boxes;  // per job lo[7], hi[7]
 };
 
-int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed) {
+int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed, int prec) {
   const uint32_t nj = (uint32_t)E.desc.size();
   out.assign(nj, JobState{});
   if (nj == 0) return G3L_OK;
@@ -315,7 +320,7 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
       EL.list = lists[k];
       EL.count = (uint32_t)std::min<uint64_t>(hcnt[k], per_kind[k]);
       if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k], s));
-      G3L_HIP(launch_eval(k, EL, s));
+      G3L_HIP(launch_eval(k, prec, EL, s));
       if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k + 1], s));
       ++launches;
       ++kind_launches[k];
@@ -475,7 +480,7 @@ int plain_eval_batch(g3l_ctx *c, int kind, int D, std::vector<HRegion> &R, size_
   EL.split = c->psplit.as<uint8_t>();
   EL.stride = cap;
   EL.njobs = 1;
-  G3L_HIP(launch_eval(kind, EL, s));
+  G3L_HIP(launch_eval(kind, G3L_FP64, EL, s));
   G3L_HIP(hipMemcpy2DAsync(hg + kFieldVal * cap, cap * sizeof(double),
                            c->pgeo.as<double>() + kFieldVal * cap, cap * sizeof(double),
                            nR * sizeof(double), 2, hipMemcpyDeviceToHost, s));
@@ -596,7 +601,7 @@ int compute_nbar(g3l_ctx *c, int first, int count) {
     }
     std::vector<JobState> st;
     g3l_batch_stats keep = c->last;
-    int rc = run_engine(c, E, st, false);
+    int rc = run_engine(c, E, st, false, G3L_FP64);
     c->last = keep;
     if (rc) return rc;
     std::vector<double> nbv(st.size());
@@ -685,7 +690,8 @@ int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
   T.nb = nb;
   T.zmin = c->zmin; T.zmax = c->zmax; T.kmin = c->kmin; T.kmax = c->kmax;
   T.mmin = c->mmin; T.mmax = c->mmax;
-  T.inv_dz = nb / (c->zmax - c->zmin);
+  T.lmr = std::log(c->mmax / c->mmin);
+  T.lkr = std::log(c->kmax / c->kmin);
   T.g = d + og; T.p1 = d + op1; T.p2 = d + op2; T.w = d + ow; T.dwdz = d + odw;
   T.rho = d + orho; T.hmf = d + ohmf; T.plin = d + opl; T.bh = d + obh; T.conc = d + oc;
   *out = c;
@@ -798,7 +804,7 @@ int g3l_eval_plain(g3l_ctx *c, const g3l_request *r, g3l_result *out) {
 int g3l_eval_batch(g3l_ctx *c, const g3l_request *reqs, size_t n, int32_t precision,
                    g3l_result *out) {
   if (!c || (!reqs && n) || (!out && n)) return G3L_EINVAL;
-  if (precision != G3L_FP64) return G3L_EINVAL;
+  if (precision != G3L_FP64 && precision != G3L_MIXED) return G3L_EINVAL;
   EngineJobs E;
   E.desc.reserve(n);
   E.args.reserve(n);
@@ -822,7 +828,7 @@ int g3l_eval_batch(g3l_ctx *c, const g3l_request *reqs, size_t n, int32_t precis
     E.boxes.insert(E.boxes.end(), box, box + 2 * kMaxDim);
   }
   std::vector<JobState> st;
-  int rc = run_engine(c, E, st, true);
+  int rc = run_engine(c, E, st, true, precision);
   if (rc) return rc;
   for (size_t i = 0; i < n; ++i) {
     out[i].raw = st[i].val;
@@ -836,9 +842,10 @@ int g3l_eval_batch(g3l_ctx *c, const g3l_request *reqs, size_t n, int32_t precis
 }
 
 
-int g3l_integrand(g3l_ctx *c, const g3l_request *r, size_t npts, const double *x,
-                  double *f) {
+int g3l_integrand(g3l_ctx *c, const g3l_request *r, int32_t precision, size_t npts,
+                  const double *x, double *f) {
   if (!c || !r || (npts && (!x || !f)) || npts > 0xffffffffull) return G3L_EINVAL;
+  if (precision != G3L_FP64 && precision != G3L_MIXED) return G3L_EINVAL;
   int rc;
   int kind;
   TermArgs a{};
@@ -864,7 +871,7 @@ int g3l_integrand(g3l_ctx *c, const g3l_request *r, size_t npts, const double *x
   rc = G3L_OK;
   if (hipMemcpy(dx.p, x, npts * D * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(da.p, &a, sizeof(a), hipMemcpyHostToDevice) != hipSuccess ||
-      launch_points(kind, c->T, da.as<TermArgs>(), dx.as<double>(), (uint32_t)npts,
+      launch_points(kind, precision, c->T, da.as<TermArgs>(), dx.as<double>(), (uint32_t)npts,
                     df.as<double>(), c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess ||
       hipMemcpy(f, df.p, npts * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)

@@ -44,12 +44,12 @@ struct SelectLaunch {
   uint32_t list_cap[kNumKinds];
 };
 
-hipError_t launch_eval(int kind, const EvalLaunch &L, hipStream_t s);
+hipError_t launch_eval(int kind, int prec, const EvalLaunch &L, hipStream_t s);
 hipError_t launch_roots(const RootsLaunch &L, hipStream_t s);
 hipError_t launch_select(const SelectLaunch &L, uint32_t njobs, hipStream_t s);
 hipError_t launch_zero(uint32_t *p, int n, hipStream_t s);
 int eval_group_size(int kind);
-hipError_t launch_points(int kind, const DevTables &T, const TermArgs *args, const double *x,
-                         uint32_t npts, double *f, hipStream_t s);
+hipError_t launch_points(int kind, int prec, const DevTables &T, const TermArgs *args,
+                         const double *x, uint32_t npts, double *f, hipStream_t s);
 
 }  // namespace g3l

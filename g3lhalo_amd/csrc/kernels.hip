@@ -22,24 +22,23 @@
 namespace g3l {
 
 // ------------------------------------------------------------ evaluation
-template <int KIND>
+template <int KIND, class P>
 __device__ __forceinline__ double integrand(const DevTables &T, const TermArgs &a,
                                             const double *x) {
   if constexpr (KIND == kNbar) {
-    DevHod h = a.h1;
-    return nbar_kernel(T, h, a.t1, x);
+    return nbar_kernel<P>(T, a, x);
   } else if constexpr (KIND == kNN1) {
-    return nnmap_1h(T, a, x);
+    return nnmap_1h<P>(T, a, x);
   } else if constexpr (KIND == kNN2) {
-    return nnmap_2h(T, a, x);
+    return nnmap_2h<P>(T, a, x);
   } else if constexpr (KIND == kNN3) {
-    return nnmap_3h(T, a, x);
+    return nnmap_3h<P>(T, a, x);
   } else if constexpr (KIND == kNM1) {
-    return nmapmap_1h(T, a, x);
+    return nmapmap_1h<P>(T, a, x);
   } else if constexpr (KIND == kNM2) {
-    return nmapmap_2h(T, a, x);
+    return nmapmap_2h<P>(T, a, x);
   } else {
-    return nmapmap_3h(T, a, x);
+    return nmapmap_3h<P>(T, a, x);
   }
 }
 
@@ -50,17 +49,17 @@ struct Group {
   static constexpr int G = D == 1 ? 17 : D == 5 ? 11 : D == 6 ? 12 : 16;
 };
 
-template <int KIND>
+template <int KIND, class P>
 __global__ __launch_bounds__(256) void k_eval(EvalLaunch L) {
   constexpr int D = kind_dim(KIND);
-  constexpr int P = Group<D>::P;
+  constexpr int NP = Group<D>::P;
   constexpr int G = Group<D>::G;
   __shared__ double s_c[G][D];
   __shared__ double s_h[G][D];
   __shared__ double s_vol[G];
   __shared__ uint32_t s_slot[G];
   __shared__ TermArgs s_args[G];
-  __shared__ double s_f[G * P];
+  __shared__ double s_f[G * NP];
 
   const int tid = threadIdx.x;
   const uint32_t first = blockIdx.x * G;
@@ -88,19 +87,19 @@ __global__ __launch_bounds__(256) void k_eval(EvalLaunch L) {
   __syncthreads();
 
   const DevTables T = L.T;
-  for (int p = tid; p < ng * P; p += 256) {
-    int g = p / P, q = p - g * P;
+  for (int p = tid; p < ng * NP; p += 256) {
+    int g = p / NP, q = p - g * NP;
     if (s_slot[g] == 0xffffffffu) continue;
     double x[D];
     rule_point<D>(q, s_c[g], s_h[g], x);
-    s_f[p] = integrand<KIND>(T, s_args[g], x);
+    s_f[p] = integrand<KIND, P>(T, s_args[g], x);
   }
   __syncthreads();
 
   if (tid < ng && s_slot[tid] != 0xffffffffu) {
     double val, err;
     int split = 0;
-    const double *v = s_f + tid * P;
+    const double *v = s_f + tid * NP;
     if constexpr (D == 1) {
       gk_reduce(v, s_h[tid][0], val, err);
     } else {
@@ -442,7 +441,7 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
 // ---------------------------------------------------------- point values
 // The integrand at caller-given points x[npts][dim] (one thread per point),
 // the device counterpart of the reference's integrand callbacks.
-template <int KIND>
+template <int KIND, class P>
 __global__ void k_points(DevTables T, const TermArgs *args, const double *x, uint32_t npts,
                          double *f) {
   constexpr int D = kind_dim(KIND);
@@ -451,45 +450,63 @@ __global__ void k_points(DevTables T, const TermArgs *args, const double *x, uin
   double p[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) p[d] = x[(size_t)i * D + d];
-  f[i] = integrand<KIND>(T, *args, p);
+  f[i] = integrand<KIND, P>(T, *args, p);
 }
 
-hipError_t launch_points(int kind, const DevTables &T, const TermArgs *args, const double *x,
-                         uint32_t npts, double *f, hipStream_t s) {
-  if (npts == 0) return hipSuccess;
-  dim3 g((npts + 255) / 256), b(256);
+template <class P>
+static void points_kind(int kind, dim3 g, dim3 b, hipStream_t s, const DevTables &T,
+                        const TermArgs *args, const double *x, uint32_t npts, double *f) {
   switch (kind) {
-    case kNbar: hipLaunchKernelGGL(k_points<kNbar>, g, b, 0, s, T, args, x, npts, f); break;
-    case kNN1: hipLaunchKernelGGL(k_points<kNN1>, g, b, 0, s, T, args, x, npts, f); break;
-    case kNN2: hipLaunchKernelGGL(k_points<kNN2>, g, b, 0, s, T, args, x, npts, f); break;
-    case kNN3: hipLaunchKernelGGL(k_points<kNN3>, g, b, 0, s, T, args, x, npts, f); break;
-    case kNM1: hipLaunchKernelGGL(k_points<kNM1>, g, b, 0, s, T, args, x, npts, f); break;
-    case kNM2: hipLaunchKernelGGL(k_points<kNM2>, g, b, 0, s, T, args, x, npts, f); break;
-    case kNM3: hipLaunchKernelGGL(k_points<kNM3>, g, b, 0, s, T, args, x, npts, f); break;
-    default: return hipErrorInvalidValue;
+    case kNbar: hipLaunchKernelGGL((k_points<kNbar, P>), g, b, 0, s, T, args, x, npts, f); break;
+    case kNN1: hipLaunchKernelGGL((k_points<kNN1, P>), g, b, 0, s, T, args, x, npts, f); break;
+    case kNN2: hipLaunchKernelGGL((k_points<kNN2, P>), g, b, 0, s, T, args, x, npts, f); break;
+    case kNN3: hipLaunchKernelGGL((k_points<kNN3, P>), g, b, 0, s, T, args, x, npts, f); break;
+    case kNM1: hipLaunchKernelGGL((k_points<kNM1, P>), g, b, 0, s, T, args, x, npts, f); break;
+    case kNM2: hipLaunchKernelGGL((k_points<kNM2, P>), g, b, 0, s, T, args, x, npts, f); break;
+    default: hipLaunchKernelGGL((k_points<kNM3, P>), g, b, 0, s, T, args, x, npts, f); break;
   }
+}
+
+hipError_t launch_points(int kind, int prec, const DevTables &T, const TermArgs *args,
+                         const double *x, uint32_t npts, double *f, hipStream_t s) {
+  if (npts == 0) return hipSuccess;
+  if (kind < 0 || kind >= kNumKinds) return hipErrorInvalidValue;
+  dim3 g((npts + 255) / 256), b(256);
+  if (prec == 0 || kind == kNbar) points_kind<P64>(kind, g, b, s, T, args, x, npts, f);
+  else points_kind<PMix>(kind, g, b, s, T, args, x, npts, f);
   return hipGetLastError();
 }
 
 // ----------------------------------------------------------------- launch
-template <int KIND>
+template <int KIND, class P>
 static hipError_t launch_eval_kind(const EvalLaunch &L, hipStream_t s) {
   constexpr int G = Group<kind_dim(KIND)>::G;
   uint32_t blocks = (L.count + G - 1) / G;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_eval<KIND>, dim3(blocks), dim3(256), 0, s, L);
+  hipLaunchKernelGGL((k_eval<KIND, P>), dim3(blocks), dim3(256), 0, s, L);
   return hipGetLastError();
 }
 
-hipError_t launch_eval(int kind, const EvalLaunch &L, hipStream_t s) {
+hipError_t launch_eval(int kind, int prec, const EvalLaunch &L, hipStream_t s) {
+  if (prec == 0 || kind == kNbar) {
+    switch (kind) {
+      case kNbar: return launch_eval_kind<kNbar, P64>(L, s);
+      case kNN1: return launch_eval_kind<kNN1, P64>(L, s);
+      case kNN2: return launch_eval_kind<kNN2, P64>(L, s);
+      case kNN3: return launch_eval_kind<kNN3, P64>(L, s);
+      case kNM1: return launch_eval_kind<kNM1, P64>(L, s);
+      case kNM2: return launch_eval_kind<kNM2, P64>(L, s);
+      case kNM3: return launch_eval_kind<kNM3, P64>(L, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   switch (kind) {
-    case kNbar: return launch_eval_kind<kNbar>(L, s);
-    case kNN1: return launch_eval_kind<kNN1>(L, s);
-    case kNN2: return launch_eval_kind<kNN2>(L, s);
-    case kNN3: return launch_eval_kind<kNN3>(L, s);
-    case kNM1: return launch_eval_kind<kNM1>(L, s);
-    case kNM2: return launch_eval_kind<kNM2>(L, s);
-    case kNM3: return launch_eval_kind<kNM3>(L, s);
+    case kNN1: return launch_eval_kind<kNN1, PMix>(L, s);
+    case kNN2: return launch_eval_kind<kNN2, PMix>(L, s);
+    case kNN3: return launch_eval_kind<kNN3, PMix>(L, s);
+    case kNM1: return launch_eval_kind<kNM1, PMix>(L, s);
+    case kNM2: return launch_eval_kind<kNM2, PMix>(L, s);
+    case kNM3: return launch_eval_kind<kNM3, PMix>(L, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -1,14 +1,23 @@
-// physics.h -- FP64 device integrands of the NNMap / NMapMap path (gfx950).
+// physics.h -- device integrands of the NNMap / NMapMap path (gfx950).
 //
-// Each function follows the reference's expression order so that, apart from
-// the last-place differences between device and host libm, the values equal
-// the reference CPU build's.  Identical sub-expressions that the reference
-// evaluates more than once (u_NFW with equal arguments in G_gg/G_g, Ncen and
-// Nsat sharing one erf, r_200 per mass) are evaluated once: recomputing a
-// pure function yields the same bits, so the sharing changes nothing.
+// The six term integrands and the n_bar integrand are written once, as
+// templates over a precision policy:
+//   P64    FP64 throughout, following the reference's expression order
+//          (src/generalFunctions.cu:6-147, src/HOD.cu:35-94,
+//           src/NNMap_Model.cu:544-1108, src/NMapMap_Model.cu:263-660).
+//   PMix   FP32 for the transcendental bulk (the u_NFW core with its Si/Ci,
+//          erf, the exponentials of log-space masses and of the aperture
+//          filters), FP64 for everything that decides a table index or
+//          carries a large dynamic range (coordinates, masses, the nearest-
+//          index chains, table values, products, the NFW normalisation).
 //
-// Reference: src/generalFunctions.cu:6-147, src/HOD.cu:35-94,
-//            src/NNMap_Model.cu:544-1108, src/NMapMap_Model.cu:263-660.
+// Deviations of P64 from the reference that change only the last bits:
+// Si/Ci for x >= 4 by Horner's rule in 1/x^2, cbrt for pow(., 1/3), exp10
+// for the 10^x of l1, l2, shared sin/cos of one argument, and sub-expressions
+// the reference evaluates repeatedly with equal arguments evaluated once.
+// Masses keep pow(10, x) and the index chains keep the reference's exact
+// order: region centres sit exactly on rounding ties of the nearest-index
+// look-ups (the root centre maps to mass index 63.5 at 128 bins).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -22,12 +31,24 @@ namespace g3l {
 struct DevTables {
   int nb;
   double zmin, zmax, kmin, kmax, mmin, mmax;
-  double inv_dz;     // unused by the FP64 path (kept for the mixed path)
+  double lmr;    // log(mmax / mmin), computed on the host
+  double lkr;    // log(kmax / kmin), computed on the host
   const double *g, *p1, *p2, *w, *dwdz, *hmf, *plin, *bh, *conc, *rho;
 };
 
 struct DevHod {
   double f, alpha, mth, sigma, mprime, beta;
+};
+
+// Per-integral parameters seen by the integrand.
+struct TermArgs {
+  double t1, t2, t3;        // apertures [rad]
+  DevHod h1, h2;            // HODs selected by type1, type2 (NMapMap: h1)
+  const double *nb1, *nb2;  // n_bar selected by type (NMapMap: nb1)
+  double A, eps;
+  int same;                 // NsatNsat same-type branch
+  // log-space HOD constants of PMix: log(mmin/M_th), log(mmin/M'), log(mmin)
+  double lth1, lmp1, lth2, lmp2, lmmin;
 };
 
 // ---------------------------------------------------------------- indices
@@ -43,14 +64,8 @@ __device__ __forceinline__ int zix_round(const DevTables &T, double z) {
 __device__ __forceinline__ int zix_trunc(const DevTables &T, double z) {
   return clampi((int)((z - T.zmin) * T.nb / (T.zmax - T.zmin)), T.nb - 1);
 }
-__device__ __forceinline__ int mix_round(const DevTables &T, double m) {
-  return clampi((int)round(log(m / T.mmin) * T.nb / log(T.mmax / T.mmin)), T.nb - 1);
-}
-__device__ __forceinline__ int mix_trunc(const DevTables &T, double m) {
-  return clampi((int)(log(m / T.mmin) * T.nb / log(T.mmax / T.mmin)), T.nb - 1);
-}
 __device__ __forceinline__ int kix_round(const DevTables &T, double k) {
-  return (int)round(log(k / T.kmin) * T.nb / log(T.kmax / T.kmin));
+  return (int)round(log(k / T.kmin) * T.nb / T.lkr);
 }
 // P(k) look-up, linear below kmin, no upper guard (src/NNMap_Model.cu:819-839):
 // an index past the row reads on into the next row; P_lin carries nb
@@ -61,9 +76,44 @@ __device__ __forceinline__ double pk(const DevTables &T, int zi, double k) {
   return T.plin[zi * T.nb] / T.kmin * k;
 }
 
+// One halo mass at one redshift: m = 10^x with the reference's rounding, its
+// look-up indices and r_200 (src/generalFunctions.cu:32-37).
+struct Mass {
+  double m, L;   // L = log(m / mmin)
+  int ir, it;    // round and int() indices
+  double r200;
+};
+__device__ __forceinline__ Mass make_mass(const DevTables &T, double x, int zt) {
+  Mass M;
+  M.m = pow(10.0, x);
+  M.L = log(M.m / T.mmin);
+  M.ir = clampi((int)round(M.L * T.nb / T.lmr), T.nb - 1);
+  M.it = clampi((int)(M.L * T.nb / T.lmr), T.nb - 1);
+  M.r200 = cbrt(0.239 * M.m / T.rho[zt] / 200);
+  return M;
+}
+
+// NFW normalisation for one (mass, f): c = f conc[z, m] with int() indices,
+// F = ln(1+c) - c/(1+c) (src/generalFunctions.cu:14-27).
+struct Nfw {
+  double r200, c, lc, F;   // lc = ln(1+c)
+};
+__device__ __forceinline__ Nfw make_nfw(const DevTables &T, const Mass &M, int zt, double f) {
+  Nfw n;
+  n.r200 = M.r200;
+  n.c = f * T.conc[zt * T.nb + M.it];
+  n.lc = log(1. + n.c);
+  n.F = n.lc - n.c / (1. + n.c);
+  return n;
+}
+
+struct Occ {
+  double nc, ns;
+};
+
 // ----------------------------------------------------------------- Si, Ci
 // Rational approximations of src/generalFunctions.cu:44-87.
-__device__ __forceinline__ void sici(double x, double &si, double &ci) {
+__device__ __forceinline__ void sici_small(double x, double &si, double &ci) {
   double x2 = x * x;
   double x4 = x2 * x2;
   double x6 = x2 * x4;
@@ -71,117 +121,311 @@ __device__ __forceinline__ void sici(double x, double &si, double &ci) {
   double x10 = x8 * x2;
   double x12 = x6 * x6;
   double x14 = x12 * x2;
+  double a = 1 - 4.54393409816329991e-2 * x2 + 1.15457225751016682e-3 * x4 -
+             1.41018536821330254e-5 * x6 + 9.43280809438713025e-8 * x8 -
+             3.53201978997168357e-10 * x10 + 7.08240282274875911e-13 * x12 -
+             6.05338212010422477e-16 * x14;
+  double b = 1 + 1.01162145739225565e-2 * x2 + 4.99175116169755106e-5 * x4 +
+             1.55654986308745614e-7 * x6 + 3.28067571055789734e-10 * x8 +
+             4.5049097575386581e-13 * x10 + 3.21107051193712168e-16 * x12;
+  si = x * a / b;
+  a = -0.25 + 7.51851524438898291e-3 * x2 - 1.27528342240267686e-4 * x4 +
+      1.05297363846239184e-6 * x6 - 4.68889508144848019e-9 * x8 +
+      1.06480802891189243e-11 * x10 - 9.93728488857585407e-15 * x12;
+  b = 1 + 1.1592605689110735e-2 * x2 + 6.72126800814254432e-5 * x4 +
+      2.55533277086129636e-7 * x6 + 6.97071295760958946e-10 * x8 +
+      1.38536352772778619e-12 * x10 + 1.89106054713059759e-15 * x12 +
+      1.39759616731376855e-18 * x14;
+  ci = 0.5772156649 + log(x) + x2 * a / b;
+}
+
+// Auxiliary functions f(x), g(x) of the x >= 4 branch as rational functions
+// of t = 1/x^2 (the reference's coefficients, Horner's rule).
+__device__ __forceinline__ void sici_aux(double x, double &f, double &g) {
+  double t = 1.0 / (x * x);
+  double fn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
+                  -4.94701168645415959931e11, 4.94816688199951963482e12),
+                  1.00795182980368574617e13), 4.20968180571076940208e12),
+                  6.40533830574022022911e11), 4.33736238870432522765e10),
+                  1.43073403821274636888e9), 2.37750310125431834034e7),
+                  1.96396372895146869801e5), 7.44437068161936700618e2);
+  double fd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
+                  1.11535493509914254097e13, 1.43468549171581016479e13),
+                  5.06084464593475076774e12), 7.08501308149515401563e11),
+                  4.58595115847765779830e10), 1.47478952192985464958e9),
+                  2.41535670165126845144e7), 1.97865247031583951450e5),
+                  7.46437068161927678031e2);
+  double gn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
+                  -1.36517137670871689e12, 6.43291613143049485e12),
+                  1.81004487464664575e13), 7.57664583257834349e12),
+                  1.09049528450362786e12), 6.83052205423625007e10),
+                  2.06297595146763354e9), 3.12557570795778731e7),
+                  2.35239181626478200e5), 8.1359520115168615e2);
+  double gd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
+                  3.99653257887490811e13, 4.01839087307656620e13),
+                  1.17164723371736605e13), 1.39866710696414565e12),
+                  7.87465017341829930e10), 2.23355543278099360e9),
+                  3.26026661647090822e7), 2.40036752835578777e5),
+                  8.19595201151451564e2);
+  f = (1 + t * fn) / ((1 + t * fd) * x);
+  g = (1 + t * gn) * t / (1 + t * gd);
+}
+
+// Si and Ci at x; sx, cx = sin x, cos x (used when x >= 4).
+__device__ __forceinline__ void sici_sc(double x, double sx, double cx, double &si, double &ci) {
   if (x < 4) {
-    double a = 1 - 4.54393409816329991e-2 * x2 + 1.15457225751016682e-3 * x4 -
-               1.41018536821330254e-5 * x6 + 9.43280809438713025e-8 * x8 -
-               3.53201978997168357e-10 * x10 + 7.08240282274875911e-13 * x12 -
-               6.05338212010422477e-16 * x14;
-    double b = 1 + 1.01162145739225565e-2 * x2 + 4.99175116169755106e-5 * x4 +
-               1.55654986308745614e-7 * x6 + 3.28067571055789734e-10 * x8 +
-               4.5049097575386581e-13 * x10 + 3.21107051193712168e-16 * x12;
-    si = x * a / b;
-    a = -0.25 + 7.51851524438898291e-3 * x2 - 1.27528342240267686e-4 * x4 +
-        1.05297363846239184e-6 * x6 - 4.68889508144848019e-9 * x8 +
-        1.06480802891189243e-11 * x10 - 9.93728488857585407e-15 * x12;
-    b = 1 + 1.1592605689110735e-2 * x2 + 6.72126800814254432e-5 * x4 +
-        2.55533277086129636e-7 * x6 + 6.97071295760958946e-10 * x8 +
-        1.38536352772778619e-12 * x10 + 1.89106054713059759e-15 * x12 +
-        1.39759616731376855e-18 * x14;
-    ci = 0.5772156649 + log(x) + x2 * a / b;
+    sici_small(x, si, ci);
   } else {
-    double x16 = x8 * x8;
-    double x18 = x16 * x2;
-    double x20 = x10 * x10;
-    double cx, sx;
-    sincos(x, &sx, &cx);
-    double f = (1 + 7.44437068161936700618e2 / x2 + 1.96396372895146869801e5 / x4 +
-                2.37750310125431834034e7 / x6 + 1.43073403821274636888e9 / x8 +
-                4.33736238870432522765e10 / x10 + 6.40533830574022022911e11 / x12 +
-                4.20968180571076940208e12 / x14 + 1.00795182980368574617e13 / x16 +
-                4.94816688199951963482e12 / x18 - 4.94701168645415959931e11 / x20) /
-               (1 + 7.46437068161927678031e2 / x2 + 1.97865247031583951450e5 / x4 +
-                2.41535670165126845144e7 / x6 + 1.47478952192985464958e9 / x8 +
-                4.58595115847765779830e10 / x10 + 7.08501308149515401563e11 / x12 +
-                5.06084464593475076774e12 / x14 + 1.43468549171581016479e13 / x16 +
-                1.11535493509914254097e13 / x18) /
-               x;
-    double g = (1 + 8.1359520115168615e2 / x2 + 2.35239181626478200e5 / x4 +
-                3.12557570795778731e7 / x6 + 2.06297595146763354e9 / x8 +
-                6.83052205423625007e10 / x10 + 1.09049528450362786e12 / x12 +
-                7.57664583257834349e12 / x14 + 1.81004487464664575e13 / x16 +
-                6.43291613143049485e12 / x18 - 1.36517137670871689e12 / x20) /
-               (1 + 8.19595201151451564e2 / x2 + 2.40036752835578777e5 / x4 +
-                3.26026661647090822e7 / x6 + 2.23355543278099360e9 / x8 +
-                7.87465017341829930e10 / x10 + 1.39866710696414565e12 / x12 +
-                1.17164723371736605e13 / x14 + 4.01839087307656620e13 / x16 +
-                3.99653257887490811e13 / x18) /
-               x2;
+    double f, g;
+    sici_aux(x, f, g);
     si = 0.5 * 3.14159265358979323846 - f * cx - g * sx;
     ci = f * sx - g * cx;
   }
 }
 
-__device__ __forceinline__ double aperture(double eta) {
-  return 0.5 * eta * eta * exp(-0.5 * eta * eta);
+__device__ __forceinline__ void sici(double x, double &si, double &ci) {
+  double sx = 0, cx = 0;
+  if (x >= 4) sincos(x, &sx, &cx);
+  sici_sc(x, sx, cx, si, ci);
 }
 
-// ---------------------------------------------------------------- HOD
-// Ncen and Nsat share log(m/M_th)/sigma/sqrt2 and its erf (src/HOD.cu:35-76).
-struct Occ {
-  double nc, ns;
+// ======================================================================
+// FP64 policy
+struct P64 {
+  static constexpr bool kMixed = false;
+  __device__ static __forceinline__ double l(double x) { return exp10(x); }
+
+  // Ncen and Nsat share log(m/M_th)/sigma/sqrt2 and its erf (src/HOD.cu:35-76).
+  __device__ static __forceinline__ Occ occ(const DevHod &h, double, double, const Mass &M) {
+    double e = 1 + erf(log(M.m / h.mth) / h.sigma / 1.414213562);
+    Occ o;
+    o.nc = 0.5 * h.alpha * e;
+    o.ns = 0.5 * e * pow(M.m / h.mprime, h.beta);
+    return o;
+  }
+
+  // m^eps of NsatNsat's cross term.
+  __device__ static __forceinline__ double mpow(const Mass &M, double eps, double) {
+    return pow(M.m, eps);
+  }
+
+  // Truncated-NFW transform (src/generalFunctions.cu:6-30).
+  __device__ static __forceinline__ double u(const Nfw &n, double k) {
+    double arg1 = k * n.r200 / n.c;
+    double arg2 = arg1 * (1 + n.c);
+    double s1, c1;
+    sincos(arg1, &s1, &c1);
+    double si1, ci1, si2, ci2;
+    sici_sc(arg1, s1, c1, si1, ci1);
+    sici(arg2, si2, ci2);
+    double term1 = s1 * (si2 - si1);
+    double term2 = c1 * (ci2 - ci1);
+    double term3 = -sin(arg1 * n.c) / arg2;
+    return (term1 + term2 + term3) / n.F;
+  }
+
+  // U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42).
+  __device__ static __forceinline__ double ap(double eta) {
+    return 0.5 * eta * eta * exp(-0.5 * eta * eta);
+  }
 };
-__device__ __forceinline__ Occ occupation(const DevHod &h, double m) {
-  double e = 1 + erf(log(m / h.mth) / h.sigma / 1.414213562);
-  Occ o;
-  o.nc = 0.5 * h.alpha * e;
-  o.ns = 0.5 * e * pow(m / h.mprime, h.beta);
-  return o;
+
+// ======================================================================
+// Mixed policy: FP32 transcendental bulk.
+namespace mix {
+
+// 2^a for an FP64 argument: the integer part exactly, the fraction by exp2f.
+__device__ __forceinline__ double exp2_split(double a) {
+  double n = floor(a);
+  float f = (float)(a - n);
+  return ldexp((double)exp2f(f), (int)n);
 }
+constexpr double kLog2e = 1.4426950408889634074;
+__device__ __forceinline__ double exp_split(double a) { return exp2_split(a * kLog2e); }
+
+// sin and cos of an FP64 argument: reduction by 2 pi in FP64 (two-term
+// Cody-Waite), FP32 evaluation of the remainder.
+__device__ __forceinline__ void sincos_red(double a, float &s, float &c) {
+  const double inv2pi = 0.15915494309189533577;
+  const double p1 = 6.28318530717958623200, p2 = 2.44929359829470635445e-16;
+  double n = rint(a * inv2pi);
+  double r = fma(-n, p1, a);
+  r = fma(-n, p2, r);
+  sincosf((float)r, &s, &c);
+}
+
+// Si/Ci auxiliary functions in FP32 for x >= 4, and f(x) - 1/x from the
+// coefficient differences of its numerator and denominator (no cancellation).
+__device__ __forceinline__ void aux32(float x, float &f, float &g, float &fm) {
+  float t = 1.0f / (x * x);
+  float fn = fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t,
+                 -4.94701168645415959931e11f, 4.94816688199951963482e12f),
+                 1.00795182980368574617e13f), 4.20968180571076940208e12f),
+                 6.40533830574022022911e11f), 4.33736238870432522765e10f),
+                 1.43073403821274636888e9f), 2.37750310125431834034e7f),
+                 1.96396372895146869801e5f), 7.44437068161936700618e2f);
+  float fd = fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t,
+                 1.11535493509914254097e13f, 1.43468549171581016479e13f),
+                 5.06084464593475076774e12f), 7.08501308149515401563e11f),
+                 4.58595115847765779830e10f), 1.47478952192985464958e9f),
+                 2.41535670165126845144e7f), 1.97865247031583951450e5f),
+                 7.46437068161927678031e2f);
+  float gn = fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t,
+                 -1.36517137670871689e12f, 6.43291613143049485e12f),
+                 1.81004487464664575e13f), 7.57664583257834349e12f),
+                 1.09049528450362786e12f), 6.83052205423625007e10f),
+                 2.06297595146763354e9f), 3.12557570795778731e7f),
+                 2.35239181626478200e5f), 8.1359520115168615e2f);
+  float gd = fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t,
+                 3.99653257887490811e13f, 4.01839087307656620e13f),
+                 1.17164723371736605e13f), 1.39866710696414565e12f),
+                 7.87465017341829930e10f), 2.23355543278099360e9f),
+                 3.26026661647090822e7f), 2.40036752835578777e5f),
+                 8.19595201151451564e2f);
+  float dn = fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t,
+                 -4.94701168645415954590e11f, -6.20538246899190625000e12f),
+                 -4.26733661912124414062e12f), -8.51162840223981323242e11f),
+                 -6.79674775754933776855e10f), -2.48588769773332548141e9f),
+                 -4.40554837171082794666e7f), -3.78536003969501121901e5f),
+                 -1.46887413643708168820e3f), -1.99999999999097743952e0f);
+  float den = 1.0f + t * fd;
+  f = (1.0f + t * fn) / (den * x);
+  g = (1.0f + t * gn) * t / (1.0f + t * gd);
+  fm = t * dn / (den * x);
+}
+
+// Si(x) = x A(x^2)/B(x^2) and Cr(x) = Ci(x) - gamma - ln x = x^2 C(x^2)/D(x^2), x < 4.
+__device__ __forceinline__ void small32(float x, float &si, float &cr) {
+  float x2 = x * x;
+  float A = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2,
+                -6.05338212010422477e-16f, 7.08240282274875911e-13f),
+                -3.53201978997168357e-10f), 9.43280809438713025e-8f),
+                -1.41018536821330254e-5f), 1.15457225751016682e-3f),
+                -4.54393409816329991e-2f), 1.0f);
+  float B = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2,
+                3.21107051193712168e-16f, 4.5049097575386581e-13f),
+                3.28067571055789734e-10f), 1.55654986308745614e-7f),
+                4.99175116169755106e-5f), 1.01162145739225565e-2f), 1.0f);
+  float C = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2,
+                -9.93728488857585407e-15f, 1.06480802891189243e-11f),
+                -4.68889508144848019e-9f), 1.05297363846239184e-6f),
+                -1.27528342240267686e-4f), 7.51851524438898291e-3f), -0.25f);
+  float D = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2,
+                1.39759616731376855e-18f, 1.89106054713059759e-15f),
+                1.38536352772778619e-12f), 6.97071295760958946e-10f),
+                2.55533277086129636e-7f), 6.72126800814254432e-5f),
+                1.1592605689110735e-2f), 1.0f);
+  si = x * A / B;
+  cr = x2 * C / D;
+}
+
+// x - sin(x) without cancellation for |x| < 1 (Taylor to x^11).
+__device__ __forceinline__ float x_minus_sin(float x, float sx) {
+  if (fabsf(x) < 1.0f) {
+    float x2 = x * x;
+    return x * x2 *
+           fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 2.5052108e-8f, -2.7557319e-6f), 1.9841270e-4f),
+                         -8.3333333e-3f), 0.16666667f);
+  }
+  return x - sx;
+}
+
+}  // namespace mix
+
+struct PMix {
+  static constexpr bool kMixed = true;
+  __device__ static __forceinline__ double l(double x) { return exp10(x); }
+
+  // log(m/M_th) = L + log(mmin/M_th); 1 + erf(X) = erfc(-X) in FP32;
+  // (m/M')^beta = exp(beta (L + log(mmin/M'))).
+  __device__ static __forceinline__ Occ occ(const DevHod &h, double lth, double lmp,
+                                            const Mass &M) {
+    double X = (M.L + lth) / h.sigma / 1.414213562;
+    double e = (double)erfcf(-(float)X);
+    Occ o;
+    o.nc = 0.5 * h.alpha * e;
+    o.ns = 0.5 * e * mix::exp_split(h.beta * (M.L + lmp));
+    return o;
+  }
+
+  __device__ static __forceinline__ double mpow(const Mass &M, double eps, double lmmin) {
+    return mix::exp_split(eps * (M.L + lmmin));
+  }
+
+  // Truncated-NFW transform in FP32 with the cancellations removed
+  // algebraically (equal to the reference's formula in exact arithmetic):
+  //   a1 >= 4:       u F = g(a1) + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c)
+  //   a2 <  4:       u = 1 + [(cos a1 - 1) ln(1+c) + (a1 c - sin a1 c)/a2
+  //                        + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si(a2) - Si(a1))] / F
+  //   a1 < 4 <= a2:  the reference's form, Si/Ci from their branches.
+  __device__ static __forceinline__ double u(const Nfw &n, double k) {
+    double a1d = k * n.r200 / n.c;
+    double a2d = a1d * (1 + n.c);
+    double xd = a1d * n.c;
+    float sx, cxx;
+    mix::sincos_red(xd, sx, cxx);
+    if (a1d >= 4.0) {
+      float f1, g1, fm1, f2, g2, fm2;
+      mix::aux32((float)a1d, f1, g1, fm1);
+      mix::aux32((float)a2d, f2, g2, fm2);
+      float uF = g1 + sx * fm2 - g2 * cxx;
+      return (double)uF / n.F;
+    }
+    float s1, c1;
+    mix::sincos_red(a1d, s1, c1);
+    float a1 = (float)a1d, a2 = (float)a2d;
+    float si1, cr1;
+    mix::small32(a1, si1, cr1);
+    if (a2d < 4.0) {
+      float si2, cr2;
+      mix::small32(a2, si2, cr2);
+      float h = sinf(0.5f * a1);
+      float rest = -2.0f * h * h * (float)n.lc + mix::x_minus_sin((float)xd, sx) / a2 +
+                   c1 * (cr2 - cr1) + s1 * (si2 - si1);
+      return 1.0 + (double)rest / n.F;
+    }
+    float f2, g2, fm2, s2, c2;
+    mix::aux32(a2, f2, g2, fm2);
+    mix::sincos_red(a2d, s2, c2);
+    float si2 = 1.57079632679489662f - f2 * c2 - g2 * s2;
+    float ci2 = f2 * s2 - g2 * c2;
+    float ci1 = 0.5772156649f + logf(a1) + cr1;
+    float uF = s1 * (si2 - si1) + c1 * (ci2 - ci1) - sx / a2;
+    return (double)uF / n.F;
+  }
+
+  __device__ static __forceinline__ double ap(double eta) {
+    double e2 = 0.5 * eta * eta;
+    return e2 * mix::exp_split(-e2);
+  }
+};
+
+// ======================================================================
+// Shared structure of the integrands.
+
+template <class P>
+__device__ __forceinline__ Occ occ1(const TermArgs &a, const Mass &M) {
+  return P::occ(a.h1, a.lth1, a.lmp1, M);
+}
+template <class P>
+__device__ __forceinline__ Occ occ2(const TermArgs &a, const Mass &M) {
+  return P::occ(a.h2, a.lth2, a.lmp2, M);
+}
+
 // src/HOD.cu:78-94 with scale1 = scale2 = 1.
-__device__ __forceinline__ double nsatnsat(double ns1, double ns2, bool same, double A,
-                                           double eps, double m) {
-  if (same) return (ns1 + 1.0 * 1.0 - 1) * ns1;
-  double r = ns1 * ns2 + A * pow(m, eps) * sqrt(ns1 * ns2);
+template <class P>
+__device__ __forceinline__ double nsatnsat(const TermArgs &a, double ns1, double ns2,
+                                           const Mass &M) {
+  if (a.same) return (ns1 + 1.0 * 1.0 - 1) * ns1;
+  double r = ns1 * ns2 + a.A * P::mpow(M, a.eps, a.lmmin) * sqrt(ns1 * ns2);
   return r < 0 ? 0.0 : r;
 }
 
-// --------------------------------------------------------------- u_NFW
-// Per (m, z, f) part of the truncated-NFW transform (src/generalFunctions.cu:6-37).
-struct Nfw {
-  double r200, c, F;
-};
-__device__ __forceinline__ double r200(const DevTables &T, double m, double z) {
-  int zi = zix_trunc(T, z);
-  return pow(0.239 * m / T.rho[zi] / 200, 1. / 3.);
-}
-__device__ __forceinline__ Nfw nfw_setup(const DevTables &T, double m, double z, double f,
-                                         double r) {
-  int zi = zix_trunc(T, z);
-  int mi = mix_trunc(T, m);
-  Nfw n;
-  n.r200 = r;
-  n.c = f * T.conc[zi * T.nb + mi];
-  n.F = log(1. + n.c) - n.c / (1. + n.c);
-  return n;
-}
-__device__ __forceinline__ double unfw(const Nfw &n, double k) {
-  double arg1 = k * n.r200 / n.c;
-  double arg2 = arg1 * (1 + n.c);
-  double si1, ci1, si2, ci2;
-  sici(arg1, si1, ci1);
-  sici(arg2, si2, ci2);
-  double s1, c1;
-  sincos(arg1, &s1, &c1);
-  double term1 = s1 * (si2 - si1);
-  double term2 = c1 * (ci2 - ci1);
-  double term3 = -sin(arg1 * n.c) / arg2;
-  return (term1 + term2 + term3) / n.F;
-}
-
-// ---------------------------------------------------------- Bi_lin, F
+// F kernel with the reference's k2/k2 (src/generalFunctions.cu:89-92).
 __device__ __forceinline__ double fkern(double k1, double k2, double c) {
   return 0.7143 + 0.2857 * c * c + 0.5 * c * (k1 / k2 + k2 / k2);
 }
+// Tree-level bispectrum (src/generalFunctions.cu:94-147).
 __device__ __forceinline__ double bilin(const DevTables &T, double k1, double k2, double cosphi,
                                         double z) {
   double k3 = sqrt(k1 * k1 + k2 * k2 + 2 * k1 * k2 * cosphi);
@@ -193,199 +437,185 @@ __device__ __forceinline__ double bilin(const DevTables &T, double k1, double k2
               fkern(k2, k3, c23) * P2 * P3);
 }
 
-// ------------------------------------------------------------ term context
-// Per-integral parameters seen by the integrand.
-struct TermArgs {
-  double t1, t2, t3;        // apertures [rad]
-  DevHod h1, h2;            // HODs selected by type1, type2 (NMapMap: h1)
-  const double *nb1, *nb2;  // n_bar selected by type (NMapMap: nb1)
-  double A, eps;
-  int same;                 // NsatNsat same-type branch
+// Coordinates shared by all terms: l1, l2, l3, cos phi, z and k = l/w.
+struct Geo {
+  double l1, l2, l3, cphi, z, w, k1, k2, k3;
+  int zr, zt;
 };
+template <class P>
+__device__ __forceinline__ Geo make_geo(const DevTables &T, const double *x, int D) {
+  Geo g;
+  g.l1 = P::l(x[0]);
+  g.l2 = P::l(x[1]);
+  g.cphi = cos(x[2]);
+  g.z = x[D - 1];
+  g.zr = zix_round(T, g.z);
+  g.zt = zix_trunc(T, g.z);
+  g.w = T.w[g.zr];
+  g.l3 = sqrt(g.l1 * g.l1 + g.l2 * g.l2 + 2 * g.l1 * g.l2 * g.cphi);
+  g.k1 = g.l1 / g.w;
+  g.k2 = g.l2 / g.w;
+  g.k3 = g.l3 / g.w;
+  return g;
+}
 
 // NNMap 1-halo, x = (log10 l1, log10 l2, phi, log10 m, z) (src/NNMap_Model.cu:544-593).
+template <class P>
 __device__ __forceinline__ double nnmap_1h(const DevTables &T, const TermArgs &a,
                                            const double *x) {
-  double l1 = pow(10.0, x[0]), l2 = pow(10.0, x[1]), phi = x[2], m = pow(10.0, x[3]);
-  double z = x[4];
-  int zi = zix_round(T, z);
-  int mi = mix_round(T, m);
-  double w_ = T.w[zi];
-  double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
-  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
-  double r = r200(T, m, z);
-  Occ o1 = occupation(a.h1, m), o2 = occupation(a.h2, m);
-  double nss = nsatnsat(o1.ns, o2.ns, a.same, a.A, a.eps, m);
-  Nfw n1 = nfw_setup(T, m, z, 1, r);
-  Nfw nf1 = nfw_setup(T, m, z, a.h1.f, r);
-  Nfw nf2 = nfw_setup(T, m, z, a.h2.f, r);
-  double u3 = unfw(n1, k3);
-  double u1 = unfw(nf1, k1);
-  double u2 = unfw(nf2, k2);
-  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * T.nb + mi] * m * u3;
+  Geo G = make_geo<P>(T, x, 5);
+  Mass M = make_mass(T, x[3], G.zt);
+  const int zi = G.zr;
+  Occ o1 = occ1<P>(a, M), o2 = occ2<P>(a, M);
+  double nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+  double u3 = P::u(make_nfw(T, M, G.zt, 1), G.k3);
+  double u1 = P::u(make_nfw(T, M, G.zt, a.h1.f), G.k1);
+  double u2 = P::u(make_nfw(T, M, G.zt, a.h2.f), G.k2);
+  double m = M.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
+  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * T.nb + M.ir] * m * u3;
   B *= o1.nc * o2.ns * u2 + o2.nc * o1.ns * u1 + nss * u1 * u2;
   double b = B / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z);
-  return l1 * l2 * m * l1 * l2 * aperture(a.t1 * l1) * aperture(a.t2 * l2) *
-         aperture(a.t3 * l3) * b;
+  return l1 * l2 * m * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) * P::ap(a.t3 * G.l3) * b;
 }
 
 // NNMap 2-halo, x = (log10 l1, log10 l2, phi, log10 m1, log10 m2, z)
 // (src/NNMap_Model.cu:787-856).
+template <class P>
 __device__ __forceinline__ double nnmap_2h(const DevTables &T, const TermArgs &a,
                                            const double *x) {
-  double l1 = pow(10.0, x[0]), l2 = pow(10.0, x[1]), phi = x[2];
-  double m1 = pow(10.0, x[3]), m2 = pow(10.0, x[4]), z = x[5];
-  const int nb = T.nb;
-  int zi = zix_round(T, z);
-  int i1 = mix_round(T, m1), i2 = mix_round(T, m2);
-  double w_ = T.w[zi];
-  double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
-  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
-  double P1 = pk(T, zi, k1), P2 = pk(T, zi, k2), P3 = pk(T, zi, k3);
-  double ra = r200(T, m1, z), rb = r200(T, m2, z);
+  Geo G = make_geo<P>(T, x, 6);
+  Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt);
+  const int zi = G.zr, nb = T.nb;
+  double P1 = pk(T, zi, G.k1), P2 = pk(T, zi, G.k2), P3 = pk(T, zi, G.k3);
   // G_gg(k1, k2, m2)
-  Occ a1 = occupation(a.h1, m2), a2 = occupation(a.h2, m2);
-  double nss = nsatnsat(a1.ns, a2.ns, a.same, a.A, a.eps, m2);
-  Nfw m2f1 = nfw_setup(T, m2, z, a.h1.f, rb), m2f2 = nfw_setup(T, m2, z, a.h2.f, rb);
-  Nfw m1f1 = nfw_setup(T, m1, z, a.h1.f, ra), m1f2 = nfw_setup(T, m1, z, a.h2.f, ra);
-  double u_k1_m2_f1 = unfw(m2f1, k1), u_k2_m2_f2 = unfw(m2f2, k2);
-  double u_k1_m1_f1 = unfw(m1f1, k1), u_k2_m1_f2 = unfw(m1f2, k2);
+  Occ a1 = occ1<P>(a, M2), a2 = occ2<P>(a, M2);
+  double nss = nsatnsat<P>(a, a1.ns, a2.ns, M2);
+  double u_k1_m2_f1 = P::u(make_nfw(T, M2, G.zt, a.h1.f), G.k1);
+  double u_k2_m2_f2 = P::u(make_nfw(T, M2, G.zt, a.h2.f), G.k2);
+  double u_k1_m1_f1 = P::u(make_nfw(T, M1, G.zt, a.h1.f), G.k1);
+  double u_k2_m1_f2 = P::u(make_nfw(T, M1, G.zt, a.h2.f), G.k2);
   double Ggg = a1.nc * a2.ns * u_k2_m2_f2 + a2.nc * a1.ns * u_k1_m2_f1 +
                nss * u_k1_m2_f1 * u_k2_m2_f2;
-  Occ b1 = occupation(a.h1, m1), b2 = occupation(a.h2, m1);
+  Occ b1 = occ1<P>(a, M1), b2 = occ2<P>(a, M1);
   double G_k1_m1_h1 = b1.nc + b1.ns * u_k1_m1_f1;
   double G_k2_m2_h2 = a2.nc + a2.ns * u_k2_m2_f2;
   double G_k2_m1_h2 = b2.nc + b2.ns * u_k2_m1_f2;
   double G_k1_m2_h1 = a1.nc + a1.ns * u_k1_m2_f1;
-  double u_k3_m1 = unfw(nfw_setup(T, m1, z, 1, ra), k3);
-  double u_k3_m2 = unfw(nfw_setup(T, m2, z, 1, rb), k3);
-  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * nb + i1] *
-             T.hmf[zi * nb + i2] * T.bh[zi * nb + i1] * T.bh[zi * nb + i2] *
+  double u_k3_m1 = P::u(make_nfw(T, M1, G.zt, 1), G.k3);
+  double u_k3_m2 = P::u(make_nfw(T, M2, G.zt, 1), G.k3);
+  double m1 = M1.m, m2 = M2.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
+  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * nb + M1.ir] *
+             T.hmf[zi * nb + M2.ir] * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              (m1 * u_k3_m1 * Ggg * P3 + m2 * u_k3_m2 * G_k1_m1_h1 * G_k2_m2_h2 * P1 +
               m2 * u_k3_m2 * G_k2_m1_h2 * G_k1_m2_h1 * P2);
   double b = 1. / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z) * B;
-  return l1 * l2 * m1 * m2 * l1 * l2 * aperture(a.t1 * l1) * aperture(a.t2 * l2) *
-         aperture(a.t3 * l3) * b;
+  return l1 * l2 * m1 * m2 * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) *
+         P::ap(a.t3 * G.l3) * b;
 }
 
 // NNMap 3-halo, x = (log10 l1, log10 l2, phi, log10 m1..m3, z)
 // (src/NNMap_Model.cu:1003-1037).
+template <class P>
 __device__ __forceinline__ double nnmap_3h(const DevTables &T, const TermArgs &a,
                                            const double *x) {
-  double l1 = pow(10.0, x[0]), l2 = pow(10.0, x[1]), phi = x[2];
-  double m1 = pow(10.0, x[3]), m2 = pow(10.0, x[4]), m3 = pow(10.0, x[5]), z = x[6];
-  const int nb = T.nb;
-  int zi = zix_round(T, z);
-  int i1 = mix_round(T, m1), i2 = mix_round(T, m2), i3 = mix_round(T, m3);
-  double w_ = T.w[zi];
-  double cphi = cos(phi);
-  double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cphi);
-  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
-  double u3 = unfw(nfw_setup(T, m3, z, 1, r200(T, m3, z)), k3);
-  Occ o1 = occupation(a.h1, m1);
-  double G1 = o1.nc + o1.ns * unfw(nfw_setup(T, m1, z, a.h1.f, r200(T, m1, z)), k1);
-  Occ o2 = occupation(a.h2, m2);
-  double G2 = o2.nc + o2.ns * unfw(nfw_setup(T, m2, z, a.h2.f, r200(T, m2, z)), k2);
-  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * nb + i1] *
-             T.hmf[zi * nb + i2] * T.hmf[zi * nb + i3] * m3 * u3 * G1 * G2 *
-             bilin(T, k1, k2, cphi, z) * T.bh[zi * nb + i1] * T.bh[zi * nb + i2] *
-             T.bh[zi * nb + i3];
+  Geo G = make_geo<P>(T, x, 7);
+  Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt),
+       M3 = make_mass(T, x[5], G.zt);
+  const int zi = G.zr, nb = T.nb;
+  double u3 = P::u(make_nfw(T, M3, G.zt, 1), G.k3);
+  Occ o1 = occ1<P>(a, M1);
+  double G1 = o1.nc + o1.ns * P::u(make_nfw(T, M1, G.zt, a.h1.f), G.k1);
+  Occ o2 = occ2<P>(a, M2);
+  double G2 = o2.nc + o2.ns * P::u(make_nfw(T, M2, G.zt, a.h2.f), G.k2);
+  double m1 = M1.m, m2 = M2.m, m3 = M3.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
+  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * nb + M1.ir] *
+             T.hmf[zi * nb + M2.ir] * T.hmf[zi * nb + M3.ir] * m3 * u3 * G1 * G2 *
+             bilin(T, G.k1, G.k2, G.cphi, z) * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
+             T.bh[zi * nb + M3.ir];
   double b = B / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z);
-  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * aperture(a.t1 * l1) * aperture(a.t2 * l2) *
-         aperture(a.t3 * l3) * b;
+  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) *
+         P::ap(a.t3 * G.l3) * b;
 }
 
 // NMapMap 1-halo (src/NMapMap_Model.cu:263-301).
+template <class P>
 __device__ __forceinline__ double nmapmap_1h(const DevTables &T, const TermArgs &a,
                                              const double *x) {
-  double l1 = pow(10.0, x[0]), l2 = pow(10.0, x[1]), phi = x[2], m = pow(10.0, x[3]);
-  double z = x[4];
-  int zi = zix_round(T, z);
-  int mi = mix_round(T, m);
+  Geo G = make_geo<P>(T, x, 5);
+  Mass M = make_mass(T, x[3], G.zt);
+  const int zi = G.zr;
   double nbar = a.nb1[zi];
-  double w_ = T.w[zi];
-  double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
-  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
-  Occ o = occupation(a.h1, m);
-  double r = r200(T, m, z);
-  Nfw n1 = nfw_setup(T, m, z, 1, r);
-  double uf = unfw(nfw_setup(T, m, z, a.h1.f, r), k1);
-  double rho = T.rho[zi];
-  double B = 1. / nbar / rho / rho * T.hmf[zi * T.nb + mi] * m * m * (o.nc + o.ns * uf) *
-             unfw(n1, k2) * unfw(n1, k3);
+  Occ o = occ1<P>(a, M);
+  Nfw n1 = make_nfw(T, M, G.zt, 1);
+  double uf = P::u(make_nfw(T, M, G.zt, a.h1.f), G.k1);
+  double rho = T.rho[zi], m = M.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
+  double B = 1. / nbar / rho / rho * T.hmf[zi * T.nb + M.ir] * m * m * (o.nc + o.ns * uf) *
+             P::u(n1, G.k2) * P::u(n1, G.k3);
   double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
-  return l1 * l2 * m * l1 * l2 * aperture(a.t1 * l1) * aperture(a.t2 * l2) *
-         aperture(a.t3 * l3) * b / nbar;
+  return l1 * l2 * m * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) * P::ap(a.t3 * G.l3) * b /
+         nbar;
 }
 
 // NMapMap 2-halo (src/NMapMap_Model.cu:402-468).
+template <class P>
 __device__ __forceinline__ double nmapmap_2h(const DevTables &T, const TermArgs &a,
                                              const double *x) {
-  double l1 = pow(10.0, x[0]), l2 = pow(10.0, x[1]), phi = x[2];
-  double m1 = pow(10.0, x[3]), m2 = pow(10.0, x[4]), z = x[5];
-  const int nb = T.nb;
-  int zi = zix_round(T, z);
-  int i1 = mix_round(T, m1), i2 = mix_round(T, m2);
+  Geo G = make_geo<P>(T, x, 6);
+  Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt);
+  const int zi = G.zr, nb = T.nb;
   double nbar = a.nb1[zi];
-  double w_ = T.w[zi];
-  double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
-  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
-  double P1 = pk(T, zi, k1), P2 = pk(T, zi, k2), P3 = pk(T, zi, k3);
-  Occ o1 = occupation(a.h1, m1), o2 = occupation(a.h1, m2);
+  double P1 = pk(T, zi, G.k1), P2 = pk(T, zi, G.k2), P3 = pk(T, zi, G.k3);
+  Occ o1 = occ1<P>(a, M1), o2 = occ1<P>(a, M2);
   double f = a.h1.f;
-  double ra = r200(T, m1, z), rb = r200(T, m2, z);
-  Nfw n1 = nfw_setup(T, m1, z, 1, ra), n2 = nfw_setup(T, m2, z, 1, rb);
-  double u_k2_m1 = unfw(n1, k2), u_k3_m1 = unfw(n1, k3);
-  double u_k2_m2 = unfw(n2, k2), u_k3_m2 = unfw(n2, k3);
-  double u_k1_m2_f = unfw(nfw_setup(T, m2, z, f, rb), k1);
-  double u_k1_m1_f = unfw(nfw_setup(T, m1, z, f, ra), k1);
-  double rho = T.rho[zi];
-  double B = 1. / nbar / rho / rho * T.hmf[zi * nb + i1] * T.hmf[zi * nb + i2] *
-             T.bh[zi * nb + i1] * T.bh[zi * nb + i2] *
+  Nfw n1 = make_nfw(T, M1, G.zt, 1), n2 = make_nfw(T, M2, G.zt, 1);
+  double u_k2_m1 = P::u(n1, G.k2), u_k3_m1 = P::u(n1, G.k3);
+  double u_k2_m2 = P::u(n2, G.k2), u_k3_m2 = P::u(n2, G.k3);
+  double u_k1_m2_f = P::u(make_nfw(T, M2, G.zt, f), G.k1);
+  double u_k1_m1_f = P::u(make_nfw(T, M1, G.zt, f), G.k1);
+  double rho = T.rho[zi], m1 = M1.m, m2 = M2.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
+  double B = 1. / nbar / rho / rho * T.hmf[zi * nb + M1.ir] * T.hmf[zi * nb + M2.ir] *
+             T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              (m1 * m2 * u_k2_m1 * u_k3_m2 * (o2.nc + o2.ns * u_k1_m2_f) * P2 +
               m1 * m2 * u_k3_m1 * u_k2_m2 * (o2.nc + o2.ns * u_k1_m2_f) * P3 +
               m2 * m2 * u_k2_m1 * u_k3_m2 * (o1.nc + o1.ns * u_k1_m1_f) * P1);
   double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
-  return l1 * l2 * m1 * m2 * l1 * l2 * aperture(a.t1 * l1) * aperture(a.t2 * l2) *
-         aperture(a.t3 * l3) * b / nbar;
+  return l1 * l2 * m1 * m2 * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) *
+         P::ap(a.t3 * G.l3) * b / nbar;
 }
 
 // NMapMap 3-halo: Ncen, Nsat at m3 with u_NFW(k1, m2, f) (src/NMapMap_Model.cu:593-632).
+template <class P>
 __device__ __forceinline__ double nmapmap_3h(const DevTables &T, const TermArgs &a,
                                              const double *x) {
-  double l1 = pow(10.0, x[0]), l2 = pow(10.0, x[1]), phi = x[2];
-  double m1 = pow(10.0, x[3]), m2 = pow(10.0, x[4]), m3 = pow(10.0, x[5]), z = x[6];
-  const int nb = T.nb;
-  int zi = zix_round(T, z);
-  int i1 = mix_round(T, m1), i2 = mix_round(T, m2), i3 = mix_round(T, m3);
+  Geo G = make_geo<P>(T, x, 7);
+  Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt),
+       M3 = make_mass(T, x[5], G.zt);
+  const int zi = G.zr, nb = T.nb;
   double nbar = a.nb1[zi];
-  double w_ = T.w[zi];
-  double cphi = cos(phi);
-  double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cphi);
-  double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
-  Occ o = occupation(a.h1, m3);
-  double ra = r200(T, m1, z), rb = r200(T, m2, z);
-  double u_k2_m1 = unfw(nfw_setup(T, m1, z, 1, ra), k2);
-  double u_k3_m2 = unfw(nfw_setup(T, m2, z, 1, rb), k3);
-  double u_k1_m2_f = unfw(nfw_setup(T, m2, z, a.h1.f, rb), k1);
-  double rho = T.rho[zi];
-  double B = 1. / nbar / rho / rho * T.hmf[zi * nb + i1] * T.hmf[zi * nb + i2] *
-             T.hmf[zi * nb + i3] * T.bh[zi * nb + i1] * T.bh[zi * nb + i2] *
-             T.bh[zi * nb + i3] * m1 * m2 * u_k2_m1 * u_k3_m2 * (o.nc + o.ns * u_k1_m2_f) *
-             bilin(T, k1, k2, cphi, z);
+  Occ o = occ1<P>(a, M3);
+  double u_k2_m1 = P::u(make_nfw(T, M1, G.zt, 1), G.k2);
+  double u_k3_m2 = P::u(make_nfw(T, M2, G.zt, 1), G.k3);
+  double u_k1_m2_f = P::u(make_nfw(T, M2, G.zt, a.h1.f), G.k1);
+  double rho = T.rho[zi], m1 = M1.m, m2 = M2.m, m3 = M3.m, w_ = G.w, z = G.z;
+  double l1 = G.l1, l2 = G.l2;
+  double B = 1. / nbar / rho / rho * T.hmf[zi * nb + M1.ir] * T.hmf[zi * nb + M2.ir] *
+             T.hmf[zi * nb + M3.ir] * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
+             T.bh[zi * nb + M3.ir] * m1 * m2 * u_k2_m1 * u_k3_m2 * (o.nc + o.ns * u_k1_m2_f) *
+             bilin(T, G.k1, G.k2, G.cphi, z);
   double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
-  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * aperture(a.t1 * l1) * aperture(a.t2 * l2) *
-         aperture(a.t3 * l3) * b / nbar;
+  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) *
+         P::ap(a.t3 * G.l3) * b / nbar;
 }
 
-// n_bar integrand, x = log10 m (src/generalFunctions.cu:233-244).
-__device__ __forceinline__ double nbar_kernel(const DevTables &T, const DevHod &h, double z,
+// n_bar integrand, x = log10 m, theta slot 1 = z (src/generalFunctions.cu:233-244).
+template <class P>
+__device__ __forceinline__ double nbar_kernel(const DevTables &T, const TermArgs &a,
                                               const double *x) {
-  double m = pow(10.0, x[0]);
-  Occ o = occupation(h, m);
-  int mi = mix_round(T, m);
-  int zi = zix_round(T, z);
-  return m * T.hmf[zi * T.nb + mi] * (o.nc + o.ns);
+  double z = a.t1;
+  int zr = zix_round(T, z);
+  Mass M = make_mass(T, x[0], zix_trunc(T, z));
+  Occ o = occ1<P>(a, M);
+  return M.m * T.hmf[zr * T.nb + M.ir] * (o.nc + o.ns);
 }
 
 }  // namespace g3l

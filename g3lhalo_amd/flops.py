@@ -64,3 +64,32 @@ def kind_of(stat: int, term: int) -> int:
 
 def flops_per_eval(stat: int, term: int) -> int:
     return FLOPS_PER_EVAL[kind_of(stat, term)]
+
+
+# FP32 share of the algorithmic work of the mixed-precision integrand:
+# the u_NFW core (all but the three FP64 argument products), the exponential
+# of each aperture filter, erf and the power of each HOD evaluation, the
+# m^eps of NsatNsat.  Everything else (indices, masses, r200, concentrations,
+# table values, products) stays FP64.
+FP32_PER_EVAL = {
+    1: 3 * (UNFW - 3) + 3 * 20 + 2 * 40 + 20,
+    2: 6 * (UNFW - 3) + 3 * 20 + 4 * 40 + 20,
+    3: 3 * (UNFW - 3) + 3 * 20 + 2 * 40,
+    4: 3 * (UNFW - 3) + 3 * 20 + 1 * 40,
+    5: 6 * (UNFW - 3) + 3 * 20 + 2 * 40,
+    6: 3 * (UNFW - 3) + 3 * 20 + 1 * 40,
+    0: 0,
+}
+
+FP64_PEAK = 78.6    # TFLOP/s, MI355X vector FP64
+FP32_PEAK = 157.3   # TFLOP/s, MI355X vector FP32
+
+
+def fp64_fraction(kind: int) -> float:
+    return 1.0 - FP32_PER_EVAL[kind] / FLOPS_PER_EVAL[kind]
+
+
+def mixed_peak_tflops(kind: int) -> float:
+    """1 / (phi64 / FP64 peak + (1 - phi64) / FP32 peak)."""
+    p = fp64_fraction(kind)
+    return 1.0 / (p / FP64_PEAK + (1.0 - p) / FP32_PEAK)

@@ -122,13 +122,14 @@ int g3l_eval_batch(g3l_ctx *ctx, const g3l_request *reqs, size_t n, int32_t prec
 /* The integrand of a request at npts points x[npts][dim] (dim 5/6/7 for
  * terms 1/2/3, coordinates log10 l1, log10 l2, phi, log10 m..., z), written
  * to f[npts].  term = 0 selects the n_bar integrand (dim 1, x = log10 m,
- * theta[0] = z, type1 = HOD slot 1 or 2 of the set).  Replaces the
+ * theta[0] = z, type1 = HOD slot 1 or 2 of the set), in the given
+ * precision (g3l_precision; n_bar is always FP64).  Replaces the
  * reference's integrand callbacks NNM::integrand_1halo/2halo/3halo,
  * NMM::integrand_1halo/2halo/3halo and integrand_nbar
  * (src/NNMap_Model.h:336,347,446; src/NMapMap_Model.h:247,327,411;
  * src/generalFunctions.h:127). */
-int g3l_integrand(g3l_ctx *ctx, const g3l_request *req, size_t npts, const double *x,
-                  double *f);
+int g3l_integrand(g3l_ctx *ctx, const g3l_request *req, int32_t precision, size_t npts,
+                  const double *x, double *f);
 
 /* Optional timing of the last g3l_eval_batch: device time of the dominant
  * (evaluation) kernels, total kernel count and algorithmic evaluations. */

@@ -45,7 +45,7 @@ def test_invalid_arguments_return_status():
     out = _lib.Result()
     assert L.g3l_eval_plain(None, C.byref(r), C.byref(out)) == _lib.G3L_EINVAL
     assert L.g3l_eval_batch(None, C.byref(r), 1, 0, C.byref(out)) == _lib.G3L_EINVAL
-    assert L.g3l_integrand(None, C.byref(r), 0, None, None) == _lib.G3L_EINVAL
+    assert L.g3l_integrand(None, C.byref(r), 0, 0, None, None) == _lib.G3L_EINVAL
     st = _lib.BatchStats()
     assert L.g3l_last_batch_stats(None, C.byref(st)) == _lib.G3L_EINVAL
 

@@ -123,3 +123,43 @@ def test_batch_terms_and_mix(ctx, oracle_model):
     for req, r in zip(reqs, out):
         alone = ctx.eval_batch([req])[0]
         assert alone.value == r.value and alone.neval == r.neval
+
+
+# ------------------------------------------------------------ mixed precision
+@pytest.mark.parametrize("stat,term", TERMS)
+def test_mixed_integrand_points(ctx, oracle_model, stat, term):
+    """FP32 transcendental bulk: per-point error distribution vs the FP64 oracle."""
+    from g3lhalo_amd import _lib
+    rng = np.random.default_rng(70 + 10 * stat + term)
+    b = oracle_model.borders()
+    lo = np.array([-1, -1, 0] + [np.log10(b[2])] * term + [b[0]])
+    hi = np.array([6, 6, 6.28318] + [np.log10(b[3])] * term + [b[1]])
+    x = lo + (hi - lo) * rng.random((20000, len(lo)))
+    theta = np.array([1.0, 2.0, 4.0]) * ARCMIN
+    say(f"mixed integrand stat={stat} term={term}")
+    req = _lib.make_request(stat, term, theta, type1=1, type2=2)
+    f = ctx.integrand(req, x, precision=_lib.MIXED)
+    f0 = oracle_model.integrand(stat, term, theta, x, 1, 2)
+    assert np.all(np.isfinite(f))
+    scale = np.abs(f0).max()
+    nz = np.abs(f0) > 1e-12 * scale
+    r = rel(f[nz], f0[nz])
+    a = np.abs(f - f0) / scale
+    print(f"mixed stat={stat} term={term}: rel median {np.median(r):.2e} p99 "
+          f"{np.quantile(r, 0.99):.2e} max {r.max():.2e}; abs/scale max {a.max():.2e}")
+    assert np.median(r) < 1e-6
+    assert a.max() < 1e-5
+
+
+@pytest.mark.parametrize("theta", [(0.5, 4.0, 32.0), (2.0, 2.0, 2.0)])
+def test_mixed_batch_terms(ctx, oracle_model, theta):
+    from g3lhalo_amd import _lib
+    th = np.array(theta) * ARCMIN
+    reqs = [_lib.make_request(s, t, th, reltol=1e-4, maxeval=1000000) for s, t in TERMS]
+    say(f"mixed batch six terms theta={theta}")
+    out = ctx.eval_batch(reqs, precision=_lib.MIXED)
+    for (s, t), r in zip(TERMS, out):
+        o = oracle_model.eval_term(s, t, th, 1, 2, 1e-4, 1000000)
+        d = rel(r.value, o["value"])
+        print(f"mixed term stat={s} term={t}: {r.value:.10e} oracle {o['value']:.10e} rel {d:.2e}")
+        assert d < TOL
