@@ -123,7 +123,7 @@ def main(argv=None):
 
     f64 = run_form(_lib.FP64)
     mixed = run_form(_lib.MIXED) if not a.fp64_only else None
-    head = mixed if mixed is not None else f64
+    head = f64
     value = head["value"]
     step_ms = head["step_ms"]
     results = f64["results"]
@@ -138,8 +138,6 @@ def main(argv=None):
     plain_rate = len(pidx) / plain_s
     dev = max(abs(plain[i].value - results[i].value) / max(abs(plain[i].value), 1e-300)
               for i in pidx)
-    dev_head = max(abs(plain[i].value - head["results"][i].value) /
-                   max(abs(plain[i].value), 1e-300) for i in pidx)
 
     # roofline of the dominant evaluation kernel (largest share of device time):
     # algorithmic flops = its evaluations x frozen flops per evaluation
@@ -156,11 +154,27 @@ def main(argv=None):
     dom = int(np.argmax(kind_ms[:7]))
     dom_flops = float(kind_evals[dom]) * flops.FLOPS_PER_EVAL[dom]
     achieved = dom_flops / (kind_ms[dom] * 1e-3) / 1e12 if kind_ms[dom] > 0 else 0.0
-    peak = flops.mixed_peak_tflops(dom) if mixed is not None else FP64_VALU_PEAK_TFLOPS
-    mixed_dev = None
+    peak = FP64_VALU_PEAK_TFLOPS
+    mixed_info = None
     if mixed is not None:
-        mixed_dev = max(abs(m.value - r.value) / max(abs(r.value), 1e-300)
-                        for m, r in zip(mixed["results"], f64["results"]))
+        mdev = np.array([abs(m.value - r.value) / max(abs(r.value), 1e-300)
+                         for m, r in zip(mixed["results"], f64["results"])])
+        mk = int(np.argmax(mixed["kind_ms"][:7]))
+        mflops = float(mixed["kind_evals"][mk]) * flops.FLOPS_PER_EVAL[mk]
+        mach = mflops / (mixed["kind_ms"][mk] * 1e-3) / 1e12 if mixed["kind_ms"][mk] > 0 else 0.0
+        mixed_info = {
+            "value": round(mixed["value"], 3), "unit": "integrals/s",
+            "ms_per_step": round(float(np.mean(mixed["step_ms"])), 3),
+            "rel_diff_vs_fp64_per_integral": {
+                "p50": float(np.quantile(mdev, 0.5)), "p90": float(np.quantile(mdev, 0.9)),
+                "p99": float(np.quantile(mdev, 0.99)), "max": float(mdev.max()),
+                "frac_above_1e-4": float(np.mean(mdev > 1e-4))},
+            "roofline": {"kernel": f"k_eval<{mk},PMix> ({KIND_NAMES[mk]})",
+                         "achieved": round(mach, 3), "peak": round(flops.mixed_peak_tflops(mk), 2),
+                         "unit": "TFLOP/s",
+                         "frac": round(mach / flops.mixed_peak_tflops(mk), 4)},
+            "note": "FP32 transcendental bulk; not the headline: these integrals end at the "
+                    "evaluation cap unconverged, and FP32 noise re-orders near-tied regions"}
 
     line = {
         "metric": "NNMap integrals/sec at 1e-4 rel tol (batched vs plain, 1 GPU); p50/p99 latency",
@@ -173,28 +187,24 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if mixed is not None else "f64",
+        "dtype": "f64",
         "data": "synthetic tables (EH no-wiggle P(k), Sheth-Tormen, NFW c(m,z)) on a 128-bin grid",
-        "config": dict(desc, form=("batched device-resident, mixed precision (FP32 "
-                                   "transcendental bulk, FP64 indices/masses/sums)"
-                                   if mixed is not None else "batched device-resident, FP64")),
-        "batched_fp64": {"value": round(f64["value"], 3), "unit": "integrals/s",
-                         "ms_per_step": round(float(np.mean(f64["step_ms"])), 3)},
-        "mixed_max_rel_vs_fp64_batched": mixed_dev,
+        "config": dict(desc, form="batched device-resident, FP64"),
+        "mixed": mixed_info,
         "plain": {"value": round(plain_rate, 3), "unit": "integrals/s",
                   "sample": f"{len(pidx)} integrals of the workload",
                   "speedup_batched_over_plain": round(value / plain_rate, 2),
-                  "speedup_batched_fp64_over_plain": round(f64["value"] / plain_rate, 2),
-                  "max_rel_diff_batched_fp64_vs_plain": float(dev),
-                  "max_rel_diff_batched_vs_plain": float(dev_head)},
+                  "speedup_mixed_over_plain": (round(mixed["value"] / plain_rate, 2)
+                                               if mixed is not None else None),
+                  "max_rel_diff_batched_vs_plain": float(dev)},
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
                        "p99": round(float(np.percentile(step_ms, 99)), 3),
                        "note": "batched form: every integral of a step completes with the step"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": round(peak, 2), "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": f"k_eval<{dom},{'PMix' if mixed is not None else 'P64'}> "
-                               f"({KIND_NAMES[dom]}, fused points + integrand + rule)",
+                     "kernel": f"k_eval<{dom},P64> ({KIND_NAMES[dom]}, fused points + "
+                               "integrand + rule)",
                      "avg_launch_ms": round(float(kind_ms[dom] / max(kind_launches[dom], 1)), 4),
                      "per_kernel": per_kind},
         "evals_per_step": int(sum(int(r.neval) for r in results)),

@@ -135,6 +135,15 @@ const double *nbar_ptr(const g3l_ctx *c, int set, int which) {
   return c->nbar.as<double>() + ((size_t)set * 2 + which) * c->nb;
 }
 
+// Log-space HOD constants: log(mmin/M_th), log(mmin/M'), log(mmin).
+void fill_logs(const g3l_ctx *c, TermArgs &a) {
+  a.lmmin = std::log(c->mmin);
+  a.lth1 = std::log(c->mmin / a.h1.mth);
+  a.lmp1 = std::log(c->mmin / a.h1.mprime);
+  a.lth2 = std::log(c->mmin / a.h2.mth);
+  a.lmp2 = std::log(c->mmin / a.h2.mprime);
+}
+
 // HOD and n_bar selection by type (src/NNMap_Model.cu:437-475); NMapMap uses
 // its one HOD (slot 1).
 TermArgs make_args(const g3l_ctx *c, const g3l_request &r) {
@@ -145,7 +154,6 @@ TermArgs make_args(const g3l_ctx *c, const g3l_request &r) {
   a.t3 = r.theta[2];
   a.A = s.A;
   a.eps = s.eps;
-  a.lmmin = std::log(c->mmin);
   if (r.stat == G3L_NMAPMAP) {
     a.h1 = a.h2 = to_dev(s.h1);
     a.nb1 = a.nb2 = nbar_ptr(c, r.hodset, 0);
@@ -158,10 +166,7 @@ TermArgs make_args(const g3l_ctx *c, const g3l_request &r) {
     a.nb2 = nbar_ptr(c, r.hodset, a2 ? 0 : 1);
     a.same = (a1 == a2) || s.identical;
   }
-  a.lth1 = std::log(c->mmin / a.h1.mth);
-  a.lmp1 = std::log(c->mmin / a.h1.mprime);
-  a.lth2 = std::log(c->mmin / a.h2.mth);
-  a.lmp2 = std::log(c->mmin / a.h2.mprime);
+  fill_logs(c, a);
   return a;
 }
 
@@ -590,6 +595,7 @@ int compute_nbar(g3l_ctx *c, int first, int count) {
           TermArgs a{};
           a.t1 = c->zmin + i * (c->zmax - c->zmin) / nb;
           a.h1 = a.h2 = to_dev(which == 0 ? c->sets[s].h1 : c->sets[s].h2);
+          fill_logs(c, a);
           E.args.push_back(a);
           E.desc.push_back(d);
           double box[2 * kMaxDim] = {0};
@@ -855,6 +861,7 @@ int g3l_integrand(g3l_ctx *c, const g3l_request *r, int32_t precision, size_t np
     a.t1 = r->theta[0];
     const HodSet &s = c->sets[r->hodset];
     a.h1 = a.h2 = to_dev(r->type1 == 2 ? s.h2 : s.h1);
+    fill_logs(c, a);
   } else {
     if ((rc = check_request(c, *r))) return rc;
     kind = kind_of(r->stat, r->term);

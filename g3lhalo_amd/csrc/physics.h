@@ -12,9 +12,11 @@
 //          index chains, table values, products, the NFW normalisation).
 //
 // Deviations of P64 from the reference that change only the last bits:
-// Si/Ci for x >= 4 by Horner's rule in 1/x^2, cbrt for pow(., 1/3), exp10
-// for the 10^x of l1, l2, shared sin/cos of one argument, and sub-expressions
-// the reference evaluates repeatedly with equal arguments evaluated once.
+// Si/Ci for x >= 4 by Horner's rule in 1/x^2; u_NFW with both arguments >= 4
+// by the cancellation-free identity noted at P64::u; the HOD logarithms and
+// powers from log(m/mmin); cbrt for pow(., 1/3); exp10 for the 10^x of
+// l1, l2; shared sin/cos of one argument; sub-expressions the reference
+// evaluates repeatedly with equal arguments evaluated once.
 // Masses keep pow(10, x) and the index chains keep the reference's exact
 // order: region centres sit exactly on rounding ties of the nearest-index
 // look-ups (the root centre maps to mass index 63.5 at 128 bins).
@@ -24,6 +26,11 @@
 #include "layout.h"
 
 #pragma clang fp contract(off)
+
+// Device functions, also compiled for the host by accuracy tools.
+#ifndef G3L_HD
+#define G3L_HD __host__ __device__ __forceinline__
+#endif
 
 namespace g3l {
 
@@ -57,20 +64,20 @@ struct TermArgs {
 // to the allocated table; inside the integration box the clamp never acts,
 // it only keeps invalid input (e.g. a zero comoving distance) from reading
 // outside the tables.
-__device__ __forceinline__ int clampi(int i, int hi) { return i < 0 ? 0 : (i > hi ? hi : i); }
-__device__ __forceinline__ int zix_round(const DevTables &T, double z) {
+G3L_HD int clampi(int i, int hi) { return i < 0 ? 0 : (i > hi ? hi : i); }
+G3L_HD int zix_round(const DevTables &T, double z) {
   return clampi((int)round((z - T.zmin) * T.nb / (T.zmax - T.zmin)), T.nb - 1);
 }
-__device__ __forceinline__ int zix_trunc(const DevTables &T, double z) {
+G3L_HD int zix_trunc(const DevTables &T, double z) {
   return clampi((int)((z - T.zmin) * T.nb / (T.zmax - T.zmin)), T.nb - 1);
 }
-__device__ __forceinline__ int kix_round(const DevTables &T, double k) {
+G3L_HD int kix_round(const DevTables &T, double k) {
   return (int)round(log(k / T.kmin) * T.nb / T.lkr);
 }
 // P(k) look-up, linear below kmin, no upper guard (src/NNMap_Model.cu:819-839):
 // an index past the row reads on into the next row; P_lin carries nb
 // trailing zeros for the last row.
-__device__ __forceinline__ double pk(const DevTables &T, int zi, double k) {
+G3L_HD double pk(const DevTables &T, int zi, double k) {
   int ki = kix_round(T, k);
   if (ki >= 0) return T.plin[clampi(zi * T.nb + ki, T.nb * T.nb + T.nb - 1)];
   return T.plin[zi * T.nb] / T.kmin * k;
@@ -83,7 +90,7 @@ struct Mass {
   int ir, it;    // round and int() indices
   double r200;
 };
-__device__ __forceinline__ Mass make_mass(const DevTables &T, double x, int zt) {
+G3L_HD Mass make_mass(const DevTables &T, double x, int zt) {
   Mass M;
   M.m = pow(10.0, x);
   M.L = log(M.m / T.mmin);
@@ -98,7 +105,7 @@ __device__ __forceinline__ Mass make_mass(const DevTables &T, double x, int zt) 
 struct Nfw {
   double r200, c, lc, F;   // lc = ln(1+c)
 };
-__device__ __forceinline__ Nfw make_nfw(const DevTables &T, const Mass &M, int zt, double f) {
+G3L_HD Nfw make_nfw(const DevTables &T, const Mass &M, int zt, double f) {
   Nfw n;
   n.r200 = M.r200;
   n.c = f * T.conc[zt * T.nb + M.it];
@@ -113,7 +120,7 @@ struct Occ {
 
 // ----------------------------------------------------------------- Si, Ci
 // Rational approximations of src/generalFunctions.cu:44-87.
-__device__ __forceinline__ void sici_small(double x, double &si, double &ci) {
+G3L_HD void sici_small(double x, double &si, double &ci) {
   double x2 = x * x;
   double x4 = x2 * x2;
   double x6 = x2 * x4;
@@ -141,7 +148,7 @@ __device__ __forceinline__ void sici_small(double x, double &si, double &ci) {
 
 // Auxiliary functions f(x), g(x) of the x >= 4 branch as rational functions
 // of t = 1/x^2 (the reference's coefficients, Horner's rule).
-__device__ __forceinline__ void sici_aux(double x, double &f, double &g) {
+G3L_HD void sici_aux(double x, double &f, double &g) {
   double t = 1.0 / (x * x);
   double fn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
                   -4.94701168645415959931e11, 4.94816688199951963482e12),
@@ -171,8 +178,40 @@ __device__ __forceinline__ void sici_aux(double x, double &f, double &g) {
   g = (1 + t * gn) * t / (1 + t * gd);
 }
 
+// g(x) and f(x) - 1/x for x >= 4, the latter from the exact coefficient
+// differences of f's numerator and denominator (no cancellation).
+G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
+  double t = 1.0 / (x * x);
+  double fd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
+                  1.11535493509914254097e13, 1.43468549171581016479e13),
+                  5.06084464593475076774e12), 7.08501308149515401563e11),
+                  4.58595115847765779830e10), 1.47478952192985464958e9),
+                  2.41535670165126845144e7), 1.97865247031583951450e5),
+                  7.46437068161927678031e2);
+  double dn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
+                  -4.94701168645415954590e11, -6.20538246899190625000e12),
+                  -4.26733661912124414062e12), -8.51162840223981323242e11),
+                  -6.79674775754933776855e10), -2.48588769773332548141e9),
+                  -4.40554837171082794666e7), -3.78536003969501121901e5),
+                  -1.46887413643708168820e3), -1.99999999999097743952e0);
+  double gn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
+                  -1.36517137670871689e12, 6.43291613143049485e12),
+                  1.81004487464664575e13), 7.57664583257834349e12),
+                  1.09049528450362786e12), 6.83052205423625007e10),
+                  2.06297595146763354e9), 3.12557570795778731e7),
+                  2.35239181626478200e5), 8.1359520115168615e2);
+  double gd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
+                  3.99653257887490811e13, 4.01839087307656620e13),
+                  1.17164723371736605e13), 1.39866710696414565e12),
+                  7.87465017341829930e10), 2.23355543278099360e9),
+                  3.26026661647090822e7), 2.40036752835578777e5),
+                  8.19595201151451564e2);
+  g = (1 + t * gn) * t / (1 + t * gd);
+  fm = t * dn / ((1 + t * fd) * x);
+}
+
 // Si and Ci at x; sx, cx = sin x, cos x (used when x >= 4).
-__device__ __forceinline__ void sici_sc(double x, double sx, double cx, double &si, double &ci) {
+G3L_HD void sici_sc(double x, double sx, double cx, double &si, double &ci) {
   if (x < 4) {
     sici_small(x, si, ci);
   } else {
@@ -183,7 +222,7 @@ __device__ __forceinline__ void sici_sc(double x, double sx, double cx, double &
   }
 }
 
-__device__ __forceinline__ void sici(double x, double &si, double &ci) {
+G3L_HD void sici(double x, double &si, double &ci) {
   double sx = 0, cx = 0;
   if (x >= 4) sincos(x, &sx, &cx);
   sici_sc(x, sx, cx, si, ci);
@@ -193,26 +232,39 @@ __device__ __forceinline__ void sici(double x, double &si, double &ci) {
 // FP64 policy
 struct P64 {
   static constexpr bool kMixed = false;
-  __device__ static __forceinline__ double l(double x) { return exp10(x); }
+  G3L_HD static double l(double x) { return exp10(x); }
 
-  // Ncen and Nsat share log(m/M_th)/sigma/sqrt2 and its erf (src/HOD.cu:35-76).
-  __device__ static __forceinline__ Occ occ(const DevHod &h, double, double, const Mass &M) {
-    double e = 1 + erf(log(M.m / h.mth) / h.sigma / 1.414213562);
+  // Ncen and Nsat share log(m/M_th)/sigma/sqrt2 and its erf (src/HOD.cu:35-76);
+  // the logarithms come from L = log(m/mmin): log(m/M_th) = L + log(mmin/M_th),
+  // (m/M')^beta = exp(beta (L + log(mmin/M'))).
+  G3L_HD static Occ occ(const DevHod &h, double lth, double lmp, const Mass &M) {
+    double e = 1 + erf((M.L + lth) / h.sigma / 1.414213562);
     Occ o;
     o.nc = 0.5 * h.alpha * e;
-    o.ns = 0.5 * e * pow(M.m / h.mprime, h.beta);
+    o.ns = 0.5 * e * exp(h.beta * (M.L + lmp));
     return o;
   }
 
-  // m^eps of NsatNsat's cross term.
-  __device__ static __forceinline__ double mpow(const Mass &M, double eps, double) {
-    return pow(M.m, eps);
+  // m^eps of NsatNsat's cross term, exp(eps (L + log mmin)).
+  G3L_HD static double mpow(const Mass &M, double eps, double lmmin) {
+    return exp(eps * (M.L + lmmin));
   }
 
-  // Truncated-NFW transform (src/generalFunctions.cu:6-30).
-  __device__ static __forceinline__ double u(const Nfw &n, double k) {
+  // Truncated-NFW transform (src/generalFunctions.cu:6-30).  When both
+  // arguments are in the asymptotic branch (arg1 >= 4) the reference's sum
+  // equals g(arg1) + sin(arg1 c)(f(arg2) - 1/arg2) - g(arg2) cos(arg1 c)
+  // exactly; that form needs one sin/cos instead of three and has none of
+  // the O(1/arg) cancellation.
+  G3L_HD static double u(const Nfw &n, double k) {
     double arg1 = k * n.r200 / n.c;
     double arg2 = arg1 * (1 + n.c);
+    if (arg1 >= 4) {
+      double g1, fm1, g2, fm2, sx, cx;
+      sici_aux_gfm(arg1, g1, fm1);
+      sici_aux_gfm(arg2, g2, fm2);
+      sincos(arg1 * n.c, &sx, &cx);
+      return (g1 + sx * fm2 - g2 * cx) / n.F;
+    }
     double s1, c1;
     sincos(arg1, &s1, &c1);
     double si1, ci1, si2, ci2;
@@ -225,7 +277,7 @@ struct P64 {
   }
 
   // U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42).
-  __device__ static __forceinline__ double ap(double eta) {
+  G3L_HD static double ap(double eta) {
     return 0.5 * eta * eta * exp(-0.5 * eta * eta);
   }
 };
@@ -235,17 +287,17 @@ struct P64 {
 namespace mix {
 
 // 2^a for an FP64 argument: the integer part exactly, the fraction by exp2f.
-__device__ __forceinline__ double exp2_split(double a) {
+G3L_HD double exp2_split(double a) {
   double n = floor(a);
   float f = (float)(a - n);
   return ldexp((double)exp2f(f), (int)n);
 }
 constexpr double kLog2e = 1.4426950408889634074;
-__device__ __forceinline__ double exp_split(double a) { return exp2_split(a * kLog2e); }
+G3L_HD double exp_split(double a) { return exp2_split(a * kLog2e); }
 
 // sin and cos of an FP64 argument: reduction by 2 pi in FP64 (two-term
 // Cody-Waite), FP32 evaluation of the remainder.
-__device__ __forceinline__ void sincos_red(double a, float &s, float &c) {
+G3L_HD void sincos_red(double a, float &s, float &c) {
   const double inv2pi = 0.15915494309189533577;
   const double p1 = 6.28318530717958623200, p2 = 2.44929359829470635445e-16;
   double n = rint(a * inv2pi);
@@ -256,7 +308,7 @@ __device__ __forceinline__ void sincos_red(double a, float &s, float &c) {
 
 // Si/Ci auxiliary functions in FP32 for x >= 4, and f(x) - 1/x from the
 // coefficient differences of its numerator and denominator (no cancellation).
-__device__ __forceinline__ void aux32(float x, float &f, float &g, float &fm) {
+G3L_HD void aux32(float x, float &f, float &g, float &fm) {
   float t = 1.0f / (x * x);
   float fn = fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t, fmaf(t,
                  -4.94701168645415959931e11f, 4.94816688199951963482e12f),
@@ -295,7 +347,7 @@ __device__ __forceinline__ void aux32(float x, float &f, float &g, float &fm) {
 }
 
 // Si(x) = x A(x^2)/B(x^2) and Cr(x) = Ci(x) - gamma - ln x = x^2 C(x^2)/D(x^2), x < 4.
-__device__ __forceinline__ void small32(float x, float &si, float &cr) {
+G3L_HD void small32(float x, float &si, float &cr) {
   float x2 = x * x;
   float A = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2,
                 -6.05338212010422477e-16f, 7.08240282274875911e-13f),
@@ -319,8 +371,17 @@ __device__ __forceinline__ void small32(float x, float &si, float &cr) {
   cr = x2 * C / D;
 }
 
+// Scaled complementary error function exp(y^2) erfc(y), y > 0.
+G3L_HD float erfcx32(float y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return erfcxf(y);
+#else
+  return (float)(std::exp((double)y * y) * std::erfc((double)y));
+#endif
+}
+
 // x - sin(x) without cancellation for |x| < 1 (Taylor to x^11).
-__device__ __forceinline__ float x_minus_sin(float x, float sx) {
+G3L_HD float x_minus_sin(float x, float sx) {
   if (fabsf(x) < 1.0f) {
     float x2 = x * x;
     return x * x2 *
@@ -334,21 +395,23 @@ __device__ __forceinline__ float x_minus_sin(float x, float sx) {
 
 struct PMix {
   static constexpr bool kMixed = true;
-  __device__ static __forceinline__ double l(double x) { return exp10(x); }
+  G3L_HD static double l(double x) { return exp10(x); }
 
-  // log(m/M_th) = L + log(mmin/M_th); 1 + erf(X) = erfc(-X) in FP32;
+  // log(m/M_th) = L + log(mmin/M_th); 1 + erf(X) = erfc(-X), for X < -1/2
+  // as exp(-X^2) (split precision) times the smooth erfcx(-X) in FP32;
   // (m/M')^beta = exp(beta (L + log(mmin/M'))).
-  __device__ static __forceinline__ Occ occ(const DevHod &h, double lth, double lmp,
+  G3L_HD static Occ occ(const DevHod &h, double lth, double lmp,
                                             const Mass &M) {
     double X = (M.L + lth) / h.sigma / 1.414213562;
-    double e = (double)erfcf(-(float)X);
+    double e = X > -0.5 ? (double)erfcf(-(float)X)
+                        : mix::exp_split(-X * X) * (double)mix::erfcx32(-(float)X);
     Occ o;
     o.nc = 0.5 * h.alpha * e;
     o.ns = 0.5 * e * mix::exp_split(h.beta * (M.L + lmp));
     return o;
   }
 
-  __device__ static __forceinline__ double mpow(const Mass &M, double eps, double lmmin) {
+  G3L_HD static double mpow(const Mass &M, double eps, double lmmin) {
     return mix::exp_split(eps * (M.L + lmmin));
   }
 
@@ -357,8 +420,8 @@ struct PMix {
   //   a1 >= 4:       u F = g(a1) + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c)
   //   a2 <  4:       u = 1 + [(cos a1 - 1) ln(1+c) + (a1 c - sin a1 c)/a2
   //                        + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si(a2) - Si(a1))] / F
-  //   a1 < 4 <= a2:  the reference's form, Si/Ci from their branches.
-  __device__ static __forceinline__ double u(const Nfw &n, double k) {
+  //   a1 < 4 <= a2:  the FP64 form (the O(1) Si/Ci terms cancel there).
+  G3L_HD static double u(const Nfw &n, double k) {
     double a1d = k * n.r200 / n.c;
     double a2d = a1d * (1 + n.c);
     double xd = a1d * n.c;
@@ -384,17 +447,11 @@ struct PMix {
                    c1 * (cr2 - cr1) + s1 * (si2 - si1);
       return 1.0 + (double)rest / n.F;
     }
-    float f2, g2, fm2, s2, c2;
-    mix::aux32(a2, f2, g2, fm2);
-    mix::sincos_red(a2d, s2, c2);
-    float si2 = 1.57079632679489662f - f2 * c2 - g2 * s2;
-    float ci2 = f2 * s2 - g2 * c2;
-    float ci1 = 0.5772156649f + logf(a1) + cr1;
-    float uF = s1 * (si2 - si1) + c1 * (ci2 - ci1) - sx / a2;
-    return (double)uF / n.F;
+    // transition a1 < 4 <= a2: O(1) terms cancel down to u F, in FP64
+    return P64::u(n, k);
   }
 
-  __device__ static __forceinline__ double ap(double eta) {
+  G3L_HD static double ap(double eta) {
     double e2 = 0.5 * eta * eta;
     return e2 * mix::exp_split(-e2);
   }
@@ -404,17 +461,17 @@ struct PMix {
 // Shared structure of the integrands.
 
 template <class P>
-__device__ __forceinline__ Occ occ1(const TermArgs &a, const Mass &M) {
+G3L_HD Occ occ1(const TermArgs &a, const Mass &M) {
   return P::occ(a.h1, a.lth1, a.lmp1, M);
 }
 template <class P>
-__device__ __forceinline__ Occ occ2(const TermArgs &a, const Mass &M) {
+G3L_HD Occ occ2(const TermArgs &a, const Mass &M) {
   return P::occ(a.h2, a.lth2, a.lmp2, M);
 }
 
 // src/HOD.cu:78-94 with scale1 = scale2 = 1.
 template <class P>
-__device__ __forceinline__ double nsatnsat(const TermArgs &a, double ns1, double ns2,
+G3L_HD double nsatnsat(const TermArgs &a, double ns1, double ns2,
                                            const Mass &M) {
   if (a.same) return (ns1 + 1.0 * 1.0 - 1) * ns1;
   double r = ns1 * ns2 + a.A * P::mpow(M, a.eps, a.lmmin) * sqrt(ns1 * ns2);
@@ -422,11 +479,11 @@ __device__ __forceinline__ double nsatnsat(const TermArgs &a, double ns1, double
 }
 
 // F kernel with the reference's k2/k2 (src/generalFunctions.cu:89-92).
-__device__ __forceinline__ double fkern(double k1, double k2, double c) {
+G3L_HD double fkern(double k1, double k2, double c) {
   return 0.7143 + 0.2857 * c * c + 0.5 * c * (k1 / k2 + k2 / k2);
 }
 // Tree-level bispectrum (src/generalFunctions.cu:94-147).
-__device__ __forceinline__ double bilin(const DevTables &T, double k1, double k2, double cosphi,
+G3L_HD double bilin(const DevTables &T, double k1, double k2, double cosphi,
                                         double z) {
   double k3 = sqrt(k1 * k1 + k2 * k2 + 2 * k1 * k2 * cosphi);
   double c13 = (k2 * k2 - k3 * k3 - k1 * k1) / (2 * k1 * k3);
@@ -443,7 +500,7 @@ struct Geo {
   int zr, zt;
 };
 template <class P>
-__device__ __forceinline__ Geo make_geo(const DevTables &T, const double *x, int D) {
+G3L_HD Geo make_geo(const DevTables &T, const double *x, int D) {
   Geo g;
   g.l1 = P::l(x[0]);
   g.l2 = P::l(x[1]);
@@ -461,7 +518,7 @@ __device__ __forceinline__ Geo make_geo(const DevTables &T, const double *x, int
 
 // NNMap 1-halo, x = (log10 l1, log10 l2, phi, log10 m, z) (src/NNMap_Model.cu:544-593).
 template <class P>
-__device__ __forceinline__ double nnmap_1h(const DevTables &T, const TermArgs &a,
+G3L_HD double nnmap_1h(const DevTables &T, const TermArgs &a,
                                            const double *x) {
   Geo G = make_geo<P>(T, x, 5);
   Mass M = make_mass(T, x[3], G.zt);
@@ -481,7 +538,7 @@ __device__ __forceinline__ double nnmap_1h(const DevTables &T, const TermArgs &a
 // NNMap 2-halo, x = (log10 l1, log10 l2, phi, log10 m1, log10 m2, z)
 // (src/NNMap_Model.cu:787-856).
 template <class P>
-__device__ __forceinline__ double nnmap_2h(const DevTables &T, const TermArgs &a,
+G3L_HD double nnmap_2h(const DevTables &T, const TermArgs &a,
                                            const double *x) {
   Geo G = make_geo<P>(T, x, 6);
   Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt);
@@ -516,7 +573,7 @@ __device__ __forceinline__ double nnmap_2h(const DevTables &T, const TermArgs &a
 // NNMap 3-halo, x = (log10 l1, log10 l2, phi, log10 m1..m3, z)
 // (src/NNMap_Model.cu:1003-1037).
 template <class P>
-__device__ __forceinline__ double nnmap_3h(const DevTables &T, const TermArgs &a,
+G3L_HD double nnmap_3h(const DevTables &T, const TermArgs &a,
                                            const double *x) {
   Geo G = make_geo<P>(T, x, 7);
   Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt),
@@ -539,7 +596,7 @@ __device__ __forceinline__ double nnmap_3h(const DevTables &T, const TermArgs &a
 
 // NMapMap 1-halo (src/NMapMap_Model.cu:263-301).
 template <class P>
-__device__ __forceinline__ double nmapmap_1h(const DevTables &T, const TermArgs &a,
+G3L_HD double nmapmap_1h(const DevTables &T, const TermArgs &a,
                                              const double *x) {
   Geo G = make_geo<P>(T, x, 5);
   Mass M = make_mass(T, x[3], G.zt);
@@ -558,7 +615,7 @@ __device__ __forceinline__ double nmapmap_1h(const DevTables &T, const TermArgs 
 
 // NMapMap 2-halo (src/NMapMap_Model.cu:402-468).
 template <class P>
-__device__ __forceinline__ double nmapmap_2h(const DevTables &T, const TermArgs &a,
+G3L_HD double nmapmap_2h(const DevTables &T, const TermArgs &a,
                                              const double *x) {
   Geo G = make_geo<P>(T, x, 6);
   Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt);
@@ -585,7 +642,7 @@ __device__ __forceinline__ double nmapmap_2h(const DevTables &T, const TermArgs 
 
 // NMapMap 3-halo: Ncen, Nsat at m3 with u_NFW(k1, m2, f) (src/NMapMap_Model.cu:593-632).
 template <class P>
-__device__ __forceinline__ double nmapmap_3h(const DevTables &T, const TermArgs &a,
+G3L_HD double nmapmap_3h(const DevTables &T, const TermArgs &a,
                                              const double *x) {
   Geo G = make_geo<P>(T, x, 7);
   Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt),
@@ -609,7 +666,7 @@ __device__ __forceinline__ double nmapmap_3h(const DevTables &T, const TermArgs 
 
 // n_bar integrand, x = log10 m, theta slot 1 = z (src/generalFunctions.cu:233-244).
 template <class P>
-__device__ __forceinline__ double nbar_kernel(const DevTables &T, const TermArgs &a,
+G3L_HD double nbar_kernel(const DevTables &T, const TermArgs &a,
                                               const double *x) {
   double z = a.t1;
   int zr = zix_round(T, z);
