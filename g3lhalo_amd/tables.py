@@ -278,3 +278,64 @@ def triplets_55() -> np.ndarray:
 
 def triplets_equal_4() -> np.ndarray:
     return np.asarray([(1, 1, 1), (2, 2, 2), (4, 4, 4), (8, 8, 8)], dtype=np.float64)
+
+
+def write_reference_files(t: Tables, directory: str, hods=(FIDUCIAL_HOD,), thetas=None) -> dict:
+    """Writes the tables in the reference's ASCII input formats: 1-D files of
+    "z value" rows on the model grid plus one row at zmax (the reference's
+    Function::at returns its default at the last abscissa, so the grid's last
+    point needs a successor), 2-D files of "z m value" / "z k value" rows in
+    z-major order, a cosmology file, HOD files and a thetas file.  Values are
+    written with 17 significant digits so they read back bit for bit."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    z, m, k = t.meta["z"], t.meta["m"], t.meta["k"]
+    n = t.nbins
+    zx = np.append(t.zmin + np.arange(n, dtype=np.float64) * (t.zmax - t.zmin) / n, t.zmax)
+    files = {}
+
+    def one_d(name, y):
+        p = os.path.join(directory, name)
+        with open(p, "w") as f:
+            for xi, yi in zip(zx, np.append(y, y[-1])):
+                f.write(f"{xi:.17g} {yi:.17g}\n")
+        files[name] = p
+
+    def two_d(name, x2, y):
+        p = os.path.join(directory, name)
+        yy = y.reshape(n, n)
+        with open(p, "w") as f:
+            for i in range(n):
+                for j in range(n):
+                    f.write(f"{z[i]:.17g} {x2[j]:.17g} {yy[i, j]:.17g}\n")
+        files[name] = p
+
+    one_d("g.dat", t.g)
+    one_d("pz_lens1.dat", t.p_lens1)
+    one_d("pz_lens2.dat", t.p_lens2)
+    one_d("w.dat", t.w)
+    one_d("dwdz.dat", t.dwdz)
+    two_d("hmf.dat", m, t.hmf)
+    two_d("Plin.dat", k, t.P_lin)
+    two_d("b_h.dat", m, t.b_h)
+    two_d("concentration.dat", m, t.concentration)
+    c = t.cosmo
+    p = os.path.join(directory, "cosmo.param")
+    with open(p, "w") as f:
+        f.write(f"Omega_m {c.Omega_m!r}\nOmega_Lambda {c.Omega_Lambda!r}\nOmega_b {c.Omega_b!r}\n"
+                f"sigma8 {c.sigma8!r}\nH0 {c.H0!r}\nns {c.ns!r}\n")
+    files["cosmo.param"] = p
+    for i, h in enumerate(hods):
+        p = os.path.join(directory, f"hod{i + 1}.param")
+        f_, alpha, mth, sig, mp, beta = (float(v) for v in h)
+        with open(p, "w") as f:
+            f.write(f"M_min {mth!r}\nsigma {sig!r}\nM' {mp!r}\nbeta {beta!r}\nalpha {alpha!r}\n"
+                    f"f {f_!r}\n")
+        files[f"hod{i + 1}.param"] = p
+    if thetas is not None:
+        p = os.path.join(directory, "thetas.dat")
+        with open(p, "w") as f:
+            for row in thetas:
+                f.write(" ".join(f"{v:.17g}" for v in row) + "\n")
+        files["thetas.dat"] = p
+    return files
