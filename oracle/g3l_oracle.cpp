@@ -19,7 +19,7 @@
  *   F, Bi_lin       -> closed form including the k2/k2 quirk; asymmetry.
  *   table indices   -> synthetic tables whose entries encode their index.
  *   term integrands -> recomposition from the pinned primitives in numpy at
- *                      random points (tests/test_oracle_integrands.py).
+ *                      random points (tests/test_oracle_physics.py).
  *   n_bar(z)        -> scipy quadrature of the same nearest-index integrand.
  * The reference's integrand sources (.cu) need CUDA headers that this image
  * lacks, so they are not compiled here; the integrands are pinned by the

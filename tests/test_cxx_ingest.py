@@ -51,3 +51,13 @@ def test_params_rejects_unknown_names(probe, tmp_path):
                         repr(t.mmax)], capture_output=True, text=True)
     assert r.returncode == 1
     assert "not in the right format" in r.stderr
+
+
+def test_driver_rejects_wrong_argument_count():
+    exe = os.path.join(ROOT, "g3lhalo_amd", "bin", "calculateNNMap.x")
+    if not os.path.exists(exe):
+        pytest.skip("drivers not built")
+    # checkCmdLine exits before any device use (src/helpers.cpp)
+    r = subprocess.run([exe, "x"], capture_output=True, text=True, timeout=30)
+    assert r.returncode == 1
+    assert "Expected:23" in r.stderr and "Got:2" in r.stderr
