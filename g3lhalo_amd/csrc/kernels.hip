@@ -42,18 +42,26 @@ __device__ __forceinline__ double integrand(const DevTables &T, const TermArgs &
   }
 }
 
-// Regions per 256-thread block: G * P close to a multiple of 256.
+// G regions per block, one rule point per thread: G * P <= 1024, as close
+// to a multiple of 64 as the point counts allow (15 * 68 = 1020,
+// 93 * 11 = 1023, 149 * 6 = 894, 241 * 4 = 964).  With one point per thread
+// the integrand is straight-line code: no loop for the compiler to hoist the
+// FP64 constants of the special functions out of, which otherwise pins them
+// in registers for the whole kernel and drops occupancy to one wave per SIMD.
 template <int D>
 struct Group {
   static constexpr int P = rule_points(D);
-  static constexpr int G = D == 1 ? 17 : D == 5 ? 11 : D == 6 ? 12 : 16;
+  static constexpr int G = D == 1 ? 68 : D == 5 ? 11 : D == 6 ? 6 : 4;
+  static constexpr int kThreads = (G * P + 63) / 64 * 64;
+  static_assert(kThreads <= 1024, "one block holds every point of its regions");
 };
 
 template <int KIND, class P>
-__global__ __launch_bounds__(256) void k_eval(EvalLaunch L) {
+__global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
   constexpr int D = kind_dim(KIND);
   constexpr int NP = Group<D>::P;
   constexpr int G = Group<D>::G;
+  constexpr int NT = Group<D>::kThreads;
   __shared__ double s_c[G][D];
   __shared__ double s_h[G][D];
   __shared__ double s_vol[G];
@@ -77,7 +85,7 @@ __global__ __launch_bounds__(256) void k_eval(EvalLaunch L) {
     }
   }
   __syncthreads();
-  for (int i = tid; i < ng * D; i += 256) {
+  for (int i = tid; i < ng * D; i += NT) {
     int g = i / D, d = i - g * D;
     uint32_t s = s_slot[g];
     if (s == 0xffffffffu) continue;
@@ -87,12 +95,13 @@ __global__ __launch_bounds__(256) void k_eval(EvalLaunch L) {
   __syncthreads();
 
   const DevTables T = L.T;
-  for (int p = tid; p < ng * NP; p += 256) {
-    int g = p / NP, q = p - g * NP;
-    if (s_slot[g] == 0xffffffffu) continue;
-    double x[D];
-    rule_point<D>(q, s_c[g], s_h[g], x);
-    s_f[p] = integrand<KIND, P>(T, s_args[g], x);
+  if (tid < ng * NP) {
+    const int g = tid / NP, q = tid - g * NP;
+    if (s_slot[g] != 0xffffffffu) {
+      double x[D];
+      rule_point<D>(q, s_c[g], s_h[g], x);
+      s_f[tid] = integrand<KIND, P>(T, s_args[g], x);
+    }
   }
   __syncthreads();
 
@@ -480,10 +489,10 @@ hipError_t launch_points(int kind, int prec, const DevTables &T, const TermArgs 
 // ----------------------------------------------------------------- launch
 template <int KIND, class P>
 static hipError_t launch_eval_kind(const EvalLaunch &L, hipStream_t s) {
-  constexpr int G = Group<kind_dim(KIND)>::G;
-  uint32_t blocks = (L.count + G - 1) / G;
+  using Gr = Group<kind_dim(KIND)>;
+  uint32_t blocks = (L.count + Gr::G - 1) / Gr::G;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_eval<KIND, P>), dim3(blocks), dim3(256), 0, s, L);
+  hipLaunchKernelGGL((k_eval<KIND, P>), dim3(blocks), dim3(Gr::kThreads), 0, s, L);
   return hipGetLastError();
 }
 
