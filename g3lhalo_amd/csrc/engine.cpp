@@ -44,7 +44,7 @@ struct DevBuf {
   size_t bytes = 0;
   int ensure(size_t want) {
     if (want <= bytes) return G3L_OK;
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
     if (hipMalloc(&p, want) != hipSuccess) {
@@ -55,7 +55,7 @@ struct DevBuf {
     return G3L_OK;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -71,7 +71,7 @@ struct HostBuf {
   size_t bytes = 0;
   int ensure(size_t want) {
     if (want <= bytes) return G3L_OK;
-    if (p) hipHostFree(p);
+    if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
@@ -82,7 +82,7 @@ struct HostBuf {
     return G3L_OK;
   }
   void release() {
-    if (p) hipHostFree(p);
+    if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -663,8 +663,20 @@ int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
     delete c;
     return G3L_EHIP;
   }
-  for (auto &e : c->ev) hipEventCreate(&e);
-  for (auto &e : c->kev) hipEventCreate(&e);
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+  for (auto &e : c->kev)
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+  for (auto e : c->ev)
+    if (!e) {
+      g3l_ctx_destroy(c);
+      return G3L_EHIP;
+    }
+  for (auto e : c->kev)
+    if (!e) {
+      g3l_ctx_destroy(c);
+      return G3L_EHIP;
+    }
   const size_t n1 = nb, n2 = (size_t)nb * nb;
   const size_t total = 6 * n1 + 4 * n2 + n1;
   if (c->tables.ensure(total * sizeof(double))) {
@@ -706,7 +718,7 @@ int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
 
 int g3l_ctx_destroy(g3l_ctx *c) {
   if (!c) return G3L_EINVAL;
-  hipDeviceSynchronize();
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->tables.release();
   c->nbar.release();
   c->pool.release(); c->split.release(); c->lists.release(); c->jobs.release();
@@ -715,10 +727,10 @@ int g3l_ctx_destroy(g3l_ctx *c) {
   c->pgeo.release(); c->psplit.release(); c->plist.release(); c->pjob.release();
   c->parg.release(); c->hgeo.release(); c->hsplit.release();
   for (auto &e : c->ev)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   for (auto &e : c->kev)
-    if (e) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return G3L_OK;
 }

@@ -56,19 +56,30 @@ struct Proj {
   uint8_t rep[NP] = {};
 };
 
+// Key of point q on the dimensions of MASK: three bits per dimension.
+constexpr unsigned gm_key(int D, int q, unsigned mask) {
+  unsigned k = 0;
+  for (int d = 0; d < D; ++d)
+    if ((mask >> d) & 1u) k |= (unsigned)gm_code(D, q, d) << (3 * d);
+  return k;
+}
+
 template <int D, unsigned MASK>
 constexpr Proj<D, MASK> make_proj() {
   Proj<D, MASK> p{};
+  unsigned key[Proj<D, MASK>::NP] = {};
+  unsigned ckey[Proj<D, MASK>::NP] = {};
+  for (int q = 0; q < Proj<D, MASK>::NP; ++q) key[q] = gm_key(D, q, MASK);
   for (int q = 0; q < Proj<D, MASK>::NP; ++q) {
     int found = -1;
-    for (int c = 0; c < p.n && found < 0; ++c) {
-      bool same = true;
-      for (int d = 0; d < D; ++d)
-        if (((MASK >> d) & 1u) && gm_code(D, q, d) != gm_code(D, p.rep[c], d)) same = false;
-      if (same) found = c;
-    }
+    for (int c = 0; c < p.n; ++c)
+      if (ckey[c] == key[q]) {
+        found = c;
+        break;
+      }
     if (found < 0) {
       p.rep[p.n] = (uint8_t)q;
+      ckey[p.n] = key[q];
       found = p.n++;
     }
     p.cls[q] = (uint8_t)found;

@@ -27,98 +27,414 @@ __device__ __forceinline__ double integrand(const DevTables &T, const TermArgs &
                                             const double *x) {
   if constexpr (KIND == kNbar) {
     return nbar_kernel<P>(T, a, x);
-  } else if constexpr (KIND == kNN1) {
-    return nnmap_1h<P>(T, a, x);
-  } else if constexpr (KIND == kNN2) {
-    return nnmap_2h<P>(T, a, x);
-  } else if constexpr (KIND == kNN3) {
-    return nnmap_3h<P>(T, a, x);
-  } else if constexpr (KIND == kNM1) {
-    return nmapmap_1h<P>(T, a, x);
-  } else if constexpr (KIND == kNM2) {
-    return nmapmap_2h<P>(T, a, x);
   } else {
-    return nmapmap_3h<P>(T, a, x);
+    DirectSrc<KIND, P> s(T, a, x);
+    return integrand_src<KIND, P>(T, a, s);
   }
 }
 
-// G regions per block, one rule point per thread: G * P <= 1024, as close
-// to a multiple of 64 as the point counts allow (15 * 68 = 1020,
-// 93 * 11 = 1023, 149 * 6 = 894, 241 * 4 = 964).  With one point per thread
-// the integrand is straight-line code: no loop for the compiler to hoist the
-// FP64 constants of the special functions out of, which otherwise pins them
-// in registers for the whole kernel and drops occupancy to one wave per SIMD.
-template <int D>
-struct Group {
+// G regions per block and NT threads: every pass gives each thread one
+// item per round, in straight-line code -- no loop for the compiler to hoist
+// the FP64 constants of the special functions out of, which otherwise pins
+// them in registers for the whole kernel and drops occupancy to one wave per
+// SIMD.  G is chosen per kind so that the dominant pass fills its rounds:
+//   n_bar  (1-D): 68 regions x 15 points = 1020 of 1024 threads;
+//   1-halo (5-D): 11 x 93 = 1023 points; u_NFW pass 11 x 66 = 726;
+//   2-halo (6-D):  4 x 149 = 596 points of 640; u_NFW pass 4 x 318 = 1272
+//                  in two rounds of 640;
+//   3-halo (7-D):  4 x 241 = 964 points; u_NFW + bispectrum pass 636 + 228.
+template <int KIND>
+struct KGroup {
+  static constexpr int D = kind_dim(KIND);
   static constexpr int P = rule_points(D);
-  static constexpr int G = D == 1 ? 68 : D == 5 ? 11 : D == 6 ? 6 : 4;
-  static constexpr int kThreads = (G * P + 63) / 64 * 64;
-  static_assert(kThreads <= 1024, "one block holds every point of its regions");
+  static constexpr int G = D == 1 ? 68 : D == 5 ? 11 : 4;
+  static constexpr int NT = (G * P + 63) / 64 * 64;
+  static_assert(NT <= 1024, "one block holds every point of its regions");
 };
 
-template <int KIND, class P>
-__global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
-  constexpr int D = kind_dim(KIND);
-  constexpr int NP = Group<D>::P;
-  constexpr int G = Group<D>::G;
-  constexpr int NT = Group<D>::kThreads;
-  __shared__ double s_c[G][D];
-  __shared__ double s_h[G][D];
-  __shared__ double s_vol[G];
-  __shared__ uint32_t s_slot[G];
-  __shared__ TermArgs s_args[G];
-  __shared__ double s_f[G * NP];
+// Region staging shared by both evaluation kernels: centre, half-width,
+// volume, slot and integral parameters of the block's regions.
+template <int D, int G>
+struct Stage {
+  double c[G][D];
+  double h[G][D];
+  double vol[G];
+  uint32_t slot[G];
+  TermArgs args[G];
+};
 
+template <int D, int G, int NT>
+__device__ __forceinline__ int stage_regions(const EvalLaunch &L, Stage<D, G> &S) {
   const int tid = threadIdx.x;
   const uint32_t first = blockIdx.x * G;
-  if (first >= L.count) return;
   const int ng = (L.count - first) < (uint32_t)G ? (int)(L.count - first) : G;
-  const uint64_t S = L.stride;
-
+  const uint64_t St = L.stride;
   if (tid < ng) {
     EvalItem it = L.list[first + tid];
-    bool ok = it.slot < S && it.job < L.njobs;
-    s_slot[tid] = ok ? it.slot : 0xffffffffu;
+    bool ok = it.slot < St && it.job < L.njobs;
+    S.slot[tid] = ok ? it.slot : 0xffffffffu;
     if (ok) {
-      s_args[tid] = L.args[L.jobs[it.job].args];
-      s_vol[tid] = L.geo[kFieldVol * S + it.slot];
+      S.args[tid] = L.args[L.jobs[it.job].args];
+      S.vol[tid] = L.geo[kFieldVol * St + it.slot];
     }
   }
   __syncthreads();
   for (int i = tid; i < ng * D; i += NT) {
     int g = i / D, d = i - g * D;
-    uint32_t s = s_slot[g];
-    if (s == 0xffffffffu) continue;
-    s_c[g][d] = L.geo[(kFieldC + d) * S + s];
-    s_h[g][d] = L.geo[(kFieldH + d) * S + s];
+    uint32_t sl = S.slot[g];
+    if (sl == 0xffffffffu) continue;
+    S.c[g][d] = L.geo[(kFieldC + d) * St + sl];
+    S.h[g][d] = L.geo[(kFieldH + d) * St + sl];
   }
   __syncthreads();
+  return ng;
+}
 
+// Rule reduction of the block's regions from the point values in f.
+template <int D, int G, int NP>
+__device__ __forceinline__ void reduce_regions(const EvalLaunch &L, const Stage<D, G> &S,
+                                               const double *f, int ng) {
+  const int tid = threadIdx.x;
+  if (tid < ng && S.slot[tid] != 0xffffffffu) {
+    double val, err;
+    int split = 0;
+    const double *v = f + tid * NP;
+    if constexpr (D == 1) {
+      gk_reduce(v, S.h[tid][0], val, err);
+    } else {
+      gm_reduce<D>(v, S.vol[tid], val, err, split);
+    }
+    const uint32_t sl = S.slot[tid];
+    const uint64_t St = L.stride;
+    L.geo[kFieldVal * St + sl] = val;
+    L.geo[kFieldErr * St + sl] = err;
+    L.split[sl] = (uint8_t)split;
+  }
+}
+
+// Direct evaluation (n_bar): every point evaluates its integrand in full.
+template <int KIND, class P>
+__global__ __launch_bounds__(1024) void k_eval_direct(EvalLaunch L) {
+  constexpr int D = kind_dim(KIND);
+  constexpr int NP = KGroup<KIND>::P;
+  constexpr int G = KGroup<KIND>::G;
+  constexpr int NT = KGroup<KIND>::NT;
+  __shared__ Stage<D, G> S;
+  __shared__ double s_f[G * NP];
+  if (blockIdx.x * G >= L.count) return;
+  const int ng = stage_regions<D, G, NT>(L, S);
+  const int tid = threadIdx.x;
   const DevTables T = L.T;
   if (tid < ng * NP) {
     const int g = tid / NP, q = tid - g * NP;
-    if (s_slot[g] != 0xffffffffu) {
+    if (S.slot[g] != 0xffffffffu) {
       double x[D];
-      rule_point<D>(q, s_c[g], s_h[g], x);
-      s_f[tid] = integrand<KIND, P>(T, s_args[g], x);
+      rule_point<D>(q, S.c[g], S.h[g], x);
+      s_f[tid] = integrand<KIND, P>(T, S.args[g], x);
     }
   }
   __syncthreads();
+  reduce_regions<D, G, NP>(L, S, s_f, ng);
+}
 
-  if (tid < ng && s_slot[tid] != 0xffffffffu) {
-    double val, err;
-    int split = 0;
-    const double *v = s_f + tid * NP;
-    if constexpr (D == 1) {
-      gk_reduce(v, s_h[tid][0], val, err);
-    } else {
-      gm_reduce<D>(v, s_vol[tid], val, err, split);
-    }
-    uint32_t s = s_slot[tid];
-    L.geo[kFieldVal * S + s] = val;
-    L.geo[kFieldErr * S + s] = err;
-    L.split[s] = (uint8_t)split;
+// ---------------------------------------------------- factored evaluation
+// The NNMap/NMapMap integrands evaluated in three passes per block:
+//   1. per region, each distinct value of the one- and two-coordinate
+//      quantities: l1, l2 and their filters, cos phi, the redshift look-ups,
+//      and per mass coordinate the mass, its indices, r_200 and the HOD
+//      occupations (a function of (m, z));
+//   2. per region, each distinct value of the u_NFW calls that depend on
+//      part of the coordinates (33 of 93 points for u(k1, m, z) in 5-D) and
+//      of the tree-level bispectrum (57 of 241 points in 7-D);
+//   3. per point, the rest (u_NFW of k3 where it depends on every
+//      coordinate, the products) from the cached values.
+// Each cached value is computed by the same expressions from the same
+// coordinates as in DirectSrc, so the point values are bit-identical.
+
+struct MassC {
+  double m, L, r200;
+  int ir, it;
+  double nc1, ns1, nc2, ns2, nss;
+};
+
+template <int KIND, int I>
+struct UF {
+  static constexpr int D = Spec<KIND>::D;
+  static constexpr USpec s = Spec<KIND>::u[I];
+  static constexpr int n = ProjTab<D, s.mask>::n;
+};
+// Offset of u factor I in the flat cache, in units of regions x classes.
+template <int KIND, int I>
+struct UOff {
+  static constexpr int v = UOff<KIND, I - 1>::v + KGroup<KIND>::G * UF<KIND, I - 1>::n;
+};
+template <int KIND>
+struct UOff<KIND, 0> {
+  static constexpr int v = 0;
+};
+
+template <int KIND>
+struct Cache {
+  using S = Spec<KIND>;
+  static constexpr int D = S::D, G = KGroup<KIND>::G;
+  static constexpr int NU_TOTAL = UOff<KIND, S::NU>::v;
+  static constexpr int NB = S::has_bilin ? ProjTab<D, bilin_mask<D>()>::n : 1;
+  static_assert(ProjTab<D, kL1>::n == 7 && ProjTab<D, kL2>::n == 7 &&
+                    ProjTab<D, bit(2)>::n == 7 && ProjTab<D, bit(D - 1)>::n == 7,
+                "seven coordinates per dimension");
+  static_assert(ProjTab<D, bit(S::mdim[0]) | bit(D - 1)>::n == 17 &&
+                    ProjTab<D, bit(S::mdim[1]) | bit(D - 1)>::n == 17 &&
+                    ProjTab<D, bit(S::mdim[2]) | bit(D - 1)>::n == 17,
+                "seventeen (mass, z) classes");
+  double l1[G][7], ap1[G][7], l2[G][7], ap2[G][7], cphi[G][7], z[G][7], w[G][7];
+  int zr[G][7], zt[G][7];
+  MassC m[S::NM][G][17];
+  double u[NU_TOTAL];
+  double bl[G * NB];
+};
+
+// Per-point view of the cache (the value source of the integrands).
+template <int KIND, class P>
+struct CachedSrc {
+  using S = Spec<KIND>;
+  static constexpr int D = S::D, G = KGroup<KIND>::G;
+  const Cache<KIND> &C;
+  const TermArgs &a;
+  int g, q;
+  Geo G_;
+  __device__ __forceinline__ CachedSrc(const Cache<KIND> &C_, const TermArgs &a_, int g_, int q_)
+      : C(C_), a(a_), g(g_), q(q_) {
+    const int c1 = kProj<D, kL1>.cls[q], c2 = kProj<D, kL2>.cls[q];
+    const int cp = kProj<D, bit(2)>.cls[q], cz = kProj<D, bit(D - 1)>.cls[q];
+    G_.l1 = C.l1[g][c1];
+    G_.l2 = C.l2[g][c2];
+    G_.cphi = C.cphi[g][cp];
+    G_.z = C.z[g][cz];
+    G_.zr = C.zr[g][cz];
+    G_.zt = C.zt[g][cz];
+    G_.w = C.w[g][cz];
+    G_.l3 = sqrt(G_.l1 * G_.l1 + G_.l2 * G_.l2 + 2 * G_.l1 * G_.l2 * G_.cphi);
+    G_.k1 = G_.l1 / G_.w;
+    G_.k2 = G_.l2 / G_.w;
+    G_.k3 = G_.l3 / G_.w;
   }
+  __device__ __forceinline__ const MassC &mc(int i) const {
+    if (i == 0) return C.m[0][g][kProj<D, bit(S::mdim[0]) | bit(D - 1)>.cls[q]];
+    if (S::NM > 1 && i == 1) return C.m[S::NM > 1 ? 1 : 0][g][kProj<D, bit(S::mdim[1]) | bit(D - 1)>.cls[q]];
+    return C.m[S::NM > 2 ? 2 : 0][g][kProj<D, bit(S::mdim[2]) | bit(D - 1)>.cls[q]];
+  }
+  __device__ __forceinline__ const Geo &geo() const { return G_; }
+  __device__ __forceinline__ Mass mass(int i) const {
+    const MassC &e = mc(i);
+    Mass M;
+    M.m = e.m;
+    M.L = e.L;
+    M.ir = e.ir;
+    M.it = e.it;
+    M.r200 = e.r200;
+    return M;
+  }
+  __device__ __forceinline__ Occ occ1(int i) const {
+    const MassC &e = mc(i);
+    return Occ{e.nc1, e.ns1};
+  }
+  __device__ __forceinline__ Occ occ2(int i) const {
+    const MassC &e = mc(i);
+    return Occ{e.nc2, e.ns2};
+  }
+  __device__ __forceinline__ double nss(int i) const { return mc(i).nss; }
+  template <int I>
+  __device__ __forceinline__ double u() const {
+    using F = UF<KIND, I>;
+    return C.u[UOff<KIND, I>::v + g * F::n + kProj<D, F::s.mask>.cls[q]];
+  }
+  __device__ __forceinline__ double ap1() const { return C.ap1[g][kProj<D, kL1>.cls[q]]; }
+  __device__ __forceinline__ double ap2() const { return C.ap2[g][kProj<D, kL2>.cls[q]]; }
+  __device__ __forceinline__ double bl() const {
+    return C.bl[g * Cache<KIND>::NB + kProj<D, bilin_mask<D>()>.cls[q]];
+  }
+};
+
+// Pass 1 item i: which region and class, per wave-aligned segment.
+template <int KIND, class P>
+__device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<KIND>::D, KGroup<KIND>::G> &St,
+                                      Cache<KIND> &C, int i, int ng) {
+  using S = Spec<KIND>;
+  constexpr int D = S::D, G = KGroup<KIND>::G;
+  constexpr int S7 = (G * 7 + 63) / 64 * 64, S17 = (G * 17 + 63) / 64 * 64;
+  int seg, r;
+  if (i < 4 * S7) {
+    seg = i / S7;
+    r = i - seg * S7;
+    if (r >= G * 7) return;
+    const int g = r / 7, c = r - g * 7;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return;
+    const TermArgs &a = St.args[g];
+    double x[D];
+    if (seg == 0) {
+      rule_point<D>(kProj<D, kL1>.rep[c], St.c[g], St.h[g], x);
+      const double l = P::l(x[0]);
+      C.l1[g][c] = l;
+      C.ap1[g][c] = P::ap(a.t1 * l);
+    } else if (seg == 1) {
+      rule_point<D>(kProj<D, kL2>.rep[c], St.c[g], St.h[g], x);
+      const double l = P::l(x[1]);
+      C.l2[g][c] = l;
+      C.ap2[g][c] = P::ap(a.t2 * l);
+    } else if (seg == 2) {
+      rule_point<D>(kProj<D, bit(2)>.rep[c], St.c[g], St.h[g], x);
+      C.cphi[g][c] = cos(x[2]);
+    } else {
+      rule_point<D>(kProj<D, bit(D - 1)>.rep[c], St.c[g], St.h[g], x);
+      const double z = x[D - 1];
+      const int zr = zix_round(T, z);
+      C.z[g][c] = z;
+      C.zr[g][c] = zr;
+      C.zt[g][c] = zix_trunc(T, z);
+      C.w[g][c] = T.w[zr];
+    }
+    return;
+  }
+  r = i - 4 * S7;
+  const int slot = r / S17;
+  if (slot >= S::NM) return;
+  r -= slot * S17;
+  if (r >= G * 17) return;
+  const int g = r / 17, c = r - g * 17;
+  if (g >= ng || St.slot[g] == 0xffffffffu) return;
+  const TermArgs &a = St.args[g];
+  const int md = slot == 0 ? S::mdim[0] : slot == 1 ? S::mdim[1] : S::mdim[2];
+  int rep;
+  if (slot == 0) rep = kProj<D, bit(S::mdim[0]) | bit(D - 1)>.rep[c];
+  else if (slot == 1) rep = kProj<D, bit(S::mdim[S::NM > 1 ? 1 : 0]) | bit(D - 1)>.rep[c];
+  else rep = kProj<D, bit(S::mdim[S::NM > 2 ? 2 : 0]) | bit(D - 1)>.rep[c];
+  double x[D];
+  rule_point<D>(rep, St.c[g], St.h[g], x);
+  const int zt = zix_trunc(T, x[D - 1]);
+  const Mass M = make_mass(T, x[md], zt);
+  MassC &e = C.m[slot][g][c];
+  e.m = M.m;
+  e.L = M.L;
+  e.r200 = M.r200;
+  e.ir = M.ir;
+  e.it = M.it;
+  const Occ o1 = occ1<P>(a, M);
+  e.nc1 = o1.nc;
+  e.ns1 = o1.ns;
+  if constexpr (S::two_hods) {
+    const Occ o2 = occ2<P>(a, M);
+    e.nc2 = o2.nc;
+    e.ns2 = o2.ns;
+    if (slot == S::nss_slot) e.nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+  }
+}
+
+// Pass 2: the inputs of u-factor item i (factor I or later).
+template <int KIND, class P, int I>
+__device__ __forceinline__ bool u_inputs(const DevTables &T,
+                                         const Stage<Spec<KIND>::D, KGroup<KIND>::G> &St,
+                                         const Cache<KIND> &C, int i, int ng, int &dst, Nfw &n,
+                                         double &k) {
+  using S = Spec<KIND>;
+  if constexpr (I >= S::NU) {
+    return false;
+  } else {
+    constexpr int D = S::D;
+    using F = UF<KIND, I>;
+    constexpr int lo = UOff<KIND, I>::v, hi = UOff<KIND, I + 1>::v;
+    if (i >= hi) return u_inputs<KIND, P, I + 1>(T, St, C, i, ng, dst, n, k);
+    const int r = i - lo, g = r / F::n, c = r - g * F::n;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return false;
+    const int q = kProj<D, F::s.mask>.rep[c];
+    const int cz = kProj<D, bit(D - 1)>.cls[q];
+    const double w = C.w[g][cz];
+    double l;
+    if constexpr (F::s.kidx == 1) {
+      l = C.l1[g][kProj<D, kL1>.cls[q]];
+    } else if constexpr (F::s.kidx == 2) {
+      l = C.l2[g][kProj<D, kL2>.cls[q]];
+    } else {
+      const double l1 = C.l1[g][kProj<D, kL1>.cls[q]], l2 = C.l2[g][kProj<D, kL2>.cls[q]];
+      const double cp = C.cphi[g][kProj<D, bit(2)>.cls[q]];
+      l = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cp);
+    }
+    k = l / w;
+    const MassC &e = C.m[F::s.slot][g][kProj<D, bit(S::mdim[F::s.slot]) | bit(D - 1)>.cls[q]];
+    Mass M;
+    M.m = e.m;
+    M.L = e.L;
+    M.ir = e.ir;
+    M.it = e.it;
+    M.r200 = e.r200;
+    n = make_nfw(T, M, C.zt[g][cz], f_of(St.args[g], F::s.fsel));
+    dst = i;
+    return true;
+  }
+}
+
+template <int KIND, class P>
+__device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<KIND>::D, KGroup<KIND>::G> &St,
+                                      Cache<KIND> &C, int i, int ng) {
+  using S = Spec<KIND>;
+  constexpr int D = S::D, G = KGroup<KIND>::G;
+  constexpr int NU_TOTAL = Cache<KIND>::NU_TOTAL;
+  constexpr int UPAD = (NU_TOTAL + 63) / 64 * 64;
+  if (i < NU_TOTAL) {
+    int dst;
+    Nfw n;
+    double k;
+    if (!u_inputs<KIND, P, 0>(T, St, C, i, ng, dst, n, k)) return;
+    C.u[dst] = P::u(n, k);
+    return;
+  }
+  if constexpr (S::has_bilin) {
+    constexpr int NB = Cache<KIND>::NB;
+    const int r = i - UPAD;
+    if (r < 0 || r >= G * NB) return;
+    const int g = r / NB, c = r - g * NB;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return;
+    const int q = kProj<D, bilin_mask<D>()>.rep[c];
+    const int cz = kProj<D, bit(D - 1)>.cls[q];
+    const double w = C.w[g][cz];
+    const double k1 = C.l1[g][kProj<D, kL1>.cls[q]] / w, k2 = C.l2[g][kProj<D, kL2>.cls[q]] / w;
+    C.bl[r] = bilin(T, k1, k2, C.cphi[g][kProj<D, bit(2)>.cls[q]], C.z[g][cz]);
+  }
+}
+
+template <int KIND, class P>
+__global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
+  using S = Spec<KIND>;
+  constexpr int D = S::D;
+  constexpr int NP = KGroup<KIND>::P;
+  constexpr int G = KGroup<KIND>::G;
+  constexpr int NT = KGroup<KIND>::NT;
+  constexpr int S7 = (G * 7 + 63) / 64 * 64, S17 = (G * 17 + 63) / 64 * 64;
+  static_assert(4 * S7 + S::NM * S17 <= NT, "pass 1 fits one item per thread");
+  constexpr int P2 = (Cache<KIND>::NU_TOTAL + 63) / 64 * 64 +
+                     (S::has_bilin ? G * Cache<KIND>::NB : 0);
+  static_assert(P2 <= 2 * NT, "pass 2 fits two items per thread");
+  static_assert(G * NP <= NT, "pass 3 fits one point per thread");
+  __shared__ Stage<D, G> St;
+  __shared__ Cache<KIND> C;
+  __shared__ double s_f[G * NP];
+  if (blockIdx.x * G >= L.count) return;
+  const int ng = stage_regions<D, G, NT>(L, St);
+  const int tid = threadIdx.x;
+  const DevTables T = L.T;
+  pass1<KIND, P>(T, St, C, tid, ng);
+  __syncthreads();
+  pass2<KIND, P>(T, St, C, tid, ng);
+  if constexpr (P2 > NT) pass2<KIND, P>(T, St, C, tid + NT, ng);
+  __syncthreads();
+  if (tid < ng * NP) {
+    const int g = tid / NP, q = tid - g * NP;
+    if (St.slot[g] != 0xffffffffu) {
+      CachedSrc<KIND, P> src(C, St.args[g], g, q);
+      s_f[tid] = integrand_src<KIND, P>(T, St.args[g], src);
+    }
+  }
+  __syncthreads();
+  reduce_regions<D, G, NP>(L, St, s_f, ng);
 }
 
 // ----------------------------------------------------------------- roots
@@ -489,10 +805,14 @@ hipError_t launch_points(int kind, int prec, const DevTables &T, const TermArgs 
 // ----------------------------------------------------------------- launch
 template <int KIND, class P>
 static hipError_t launch_eval_kind(const EvalLaunch &L, hipStream_t s) {
-  using Gr = Group<kind_dim(KIND)>;
+  using Gr = KGroup<KIND>;
   uint32_t blocks = (L.count + Gr::G - 1) / Gr::G;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_eval<KIND, P>), dim3(blocks), dim3(Gr::kThreads), 0, s, L);
+  if constexpr (KIND == kNbar) {
+    hipLaunchKernelGGL((k_eval_direct<KIND, P>), dim3(blocks), dim3(Gr::NT), 0, s, L);
+  } else {
+    hipLaunchKernelGGL((k_eval<KIND, P>), dim3(blocks), dim3(Gr::NT), 0, s, L);
+  }
   return hipGetLastError();
 }
 
@@ -522,10 +842,10 @@ hipError_t launch_eval(int kind, int prec, const EvalLaunch &L, hipStream_t s) {
 
 int eval_group_size(int kind) {
   switch (kind_dim(kind)) {
-    case 1: return Group<1>::G;
-    case 5: return Group<5>::G;
-    case 6: return Group<6>::G;
-    default: return Group<7>::G;
+    case 1: return KGroup<kNbar>::G;
+    case 5: return KGroup<kNN1>::G;
+    case 6: return KGroup<kNN2>::G;
+    default: return KGroup<kNN3>::G;
   }
 }
 

@@ -23,6 +23,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "factor.h"
 #include "layout.h"
 
 #pragma clang fp contract(off)
@@ -516,119 +517,218 @@ G3L_HD Geo make_geo(const DevTables &T, const double *x, int D) {
   return g;
 }
 
-// NNMap 1-halo, x = (log10 l1, log10 l2, phi, log10 m, z) (src/NNMap_Model.cu:544-593).
-template <class P>
-G3L_HD double nnmap_1h(const DevTables &T, const TermArgs &a,
-                                           const double *x) {
-  Geo G = make_geo<P>(T, x, 5);
-  Mass M = make_mass(T, x[3], G.zt);
+// ======================================================================
+// Per-kind structure: which coordinates are halo masses, and the u_NFW
+// calls that depend on only part of the coordinates.  Coordinates:
+//   5-D (1h): l1, l2, phi, m, z;  6-D (2h): l1, l2, phi, m1, m2, z;
+//   7-D (3h): l1, l2, phi, m1, m2, m3, z  (l and m as log10).
+// USpec {mask, kidx, slot, fsel}: u_NFW(k_kidx, mass slot, f) with f = 1,
+// the first or the second HOD's f for fsel = 0, 1, 2; mask = coordinates it
+// depends on (l_kidx, the mass, z; l1, l2, phi for k3).
+struct USpec {
+  unsigned mask;
+  int kidx, slot, fsel;
+};
+constexpr unsigned kL1 = bit(0), kL2 = bit(1), kL3 = bit(0) | bit(1) | bit(2);
+
+template <int KIND>
+struct Spec;
+template <>
+struct Spec<kNN1> {
+  static constexpr int D = 5, NM = 1, NU = 2, nss_slot = 0;
+  static constexpr bool two_hods = true, has_bilin = false;
+  static constexpr int mdim[3] = {3, 0, 0};
+  static constexpr USpec u[6] = {{kL1 | bit(3) | bit(4), 1, 0, 1}, {kL2 | bit(3) | bit(4), 2, 0, 2}};
+};
+template <>
+struct Spec<kNN2> {
+  static constexpr int D = 6, NM = 2, NU = 6, nss_slot = 1;
+  static constexpr bool two_hods = true, has_bilin = false;
+  static constexpr int mdim[3] = {3, 4, 0};
+  static constexpr USpec u[6] = {{kL1 | bit(4) | bit(5), 1, 1, 1},   // u(k1, m2, f1)
+                                 {kL2 | bit(4) | bit(5), 2, 1, 2},   // u(k2, m2, f2)
+                                 {kL1 | bit(3) | bit(5), 1, 0, 1},   // u(k1, m1, f1)
+                                 {kL2 | bit(3) | bit(5), 2, 0, 2},   // u(k2, m1, f2)
+                                 {kL3 | bit(3) | bit(5), 3, 0, 0},   // u(k3, m1)
+                                 {kL3 | bit(4) | bit(5), 3, 1, 0}};  // u(k3, m2)
+};
+template <>
+struct Spec<kNN3> {
+  static constexpr int D = 7, NM = 3, NU = 3, nss_slot = -1;
+  static constexpr bool two_hods = true, has_bilin = true;
+  static constexpr int mdim[3] = {3, 4, 5};
+  static constexpr USpec u[6] = {{kL3 | bit(5) | bit(6), 3, 2, 0},   // u(k3, m3)
+                                 {kL1 | bit(3) | bit(6), 1, 0, 1},   // u(k1, m1, f1)
+                                 {kL2 | bit(4) | bit(6), 2, 1, 2}};  // u(k2, m2, f2)
+};
+template <>
+struct Spec<kNM1> {
+  static constexpr int D = 5, NM = 1, NU = 2, nss_slot = -1;
+  static constexpr bool two_hods = false, has_bilin = false;
+  static constexpr int mdim[3] = {3, 0, 0};
+  static constexpr USpec u[6] = {{kL1 | bit(3) | bit(4), 1, 0, 1},   // u(k1, m, f)
+                                 {kL2 | bit(3) | bit(4), 2, 0, 0}};  // u(k2, m)
+};
+template <>
+struct Spec<kNM2> {
+  static constexpr int D = 6, NM = 2, NU = 6, nss_slot = -1;
+  static constexpr bool two_hods = false, has_bilin = false;
+  static constexpr int mdim[3] = {3, 4, 0};
+  static constexpr USpec u[6] = {{kL2 | bit(3) | bit(5), 2, 0, 0},   // u(k2, m1)
+                                 {kL3 | bit(3) | bit(5), 3, 0, 0},   // u(k3, m1)
+                                 {kL2 | bit(4) | bit(5), 2, 1, 0},   // u(k2, m2)
+                                 {kL3 | bit(4) | bit(5), 3, 1, 0},   // u(k3, m2)
+                                 {kL1 | bit(4) | bit(5), 1, 1, 1},   // u(k1, m2, f)
+                                 {kL1 | bit(3) | bit(5), 1, 0, 1}};  // u(k1, m1, f)
+};
+template <>
+struct Spec<kNM3> {
+  static constexpr int D = 7, NM = 3, NU = 3, nss_slot = -1;
+  static constexpr bool two_hods = false, has_bilin = true;
+  static constexpr int mdim[3] = {3, 4, 5};
+  static constexpr USpec u[6] = {{kL2 | bit(3) | bit(6), 2, 0, 0},   // u(k2, m1)
+                                 {kL3 | bit(4) | bit(6), 3, 1, 0},   // u(k3, m2)
+                                 {kL1 | bit(4) | bit(6), 1, 1, 1}};  // u(k1, m2, f)
+};
+// Coordinates of the tree-level bispectrum: l1, l2, phi, z.
+template <int D>
+constexpr unsigned bilin_mask() { return kL3 | bit(D - 1); }
+
+G3L_HD double f_of(const TermArgs &a, int fsel) {
+  return fsel == 0 ? 1.0 : fsel == 1 ? a.h1.f : a.h2.f;
+}
+
+// Direct source: every quantity computed at the point itself.
+template <int KIND, class P>
+struct DirectSrc {
+  using S = Spec<KIND>;
+  const DevTables &T;
+  const TermArgs &a;
+  Geo G;
+  Mass M[S::NM];
+  G3L_HD DirectSrc(const DevTables &T_, const TermArgs &a_, const double *x) : T(T_), a(a_) {
+    G = make_geo<P>(T, x, S::D);
+#pragma unroll
+    for (int i = 0; i < S::NM; ++i) M[i] = make_mass(T, x[S::mdim[i]], G.zt);
+  }
+  G3L_HD const Geo &geo() const { return G; }
+  G3L_HD Mass mass(int i) const { return M[i]; }
+  G3L_HD Occ occ1(int i) const { return g3l::occ1<P>(a, M[i]); }
+  G3L_HD Occ occ2(int i) const { return g3l::occ2<P>(a, M[i]); }
+  G3L_HD double nss(int i) const { return nsatnsat<P>(a, occ1(i).ns, occ2(i).ns, M[i]); }
+  template <int I>
+  G3L_HD double u() const {
+    constexpr USpec s = S::u[I];
+    const double k = s.kidx == 1 ? G.k1 : s.kidx == 2 ? G.k2 : G.k3;
+    return P::u(make_nfw(T, M[s.slot], G.zt, f_of(a, s.fsel)), k);
+  }
+  G3L_HD double ap1() const { return P::ap(a.t1 * G.l1); }
+  G3L_HD double ap2() const { return P::ap(a.t2 * G.l2); }
+  G3L_HD double bl() const { return bilin(T, G.k1, G.k2, G.cphi, G.z); }
+};
+
+// The six integrands over a value source S (DirectSrc, or the kernel's
+// per-region cache); the arithmetic after the look-ups is the same for both.
+
+// NNMap 1-halo (src/NNMap_Model.cu:544-593).
+template <class P, class S>
+G3L_HD double nnmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
+  const Geo &G = s.geo();
+  const Mass M = s.mass(0);
   const int zi = G.zr;
-  Occ o1 = occ1<P>(a, M), o2 = occ2<P>(a, M);
-  double nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+  Occ o1 = s.occ1(0), o2 = s.occ2(0);
+  double nss = s.nss(0);
   double u3 = P::u(make_nfw(T, M, G.zt, 1), G.k3);
-  double u1 = P::u(make_nfw(T, M, G.zt, a.h1.f), G.k1);
-  double u2 = P::u(make_nfw(T, M, G.zt, a.h2.f), G.k2);
+  double u1 = s.template u<0>();
+  double u2 = s.template u<1>();
   double m = M.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
   double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * T.nb + M.ir] * m * u3;
   B *= o1.nc * o2.ns * u2 + o2.nc * o1.ns * u1 + nss * u1 * u2;
   double b = B / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z);
-  return l1 * l2 * m * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) * P::ap(a.t3 * G.l3) * b;
+  return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
 }
 
-// NNMap 2-halo, x = (log10 l1, log10 l2, phi, log10 m1, log10 m2, z)
-// (src/NNMap_Model.cu:787-856).
-template <class P>
-G3L_HD double nnmap_2h(const DevTables &T, const TermArgs &a,
-                                           const double *x) {
-  Geo G = make_geo<P>(T, x, 6);
-  Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt);
+// NNMap 2-halo (src/NNMap_Model.cu:787-856).
+template <class P, class S>
+G3L_HD double nnmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
+  const Geo &G = s.geo();
+  const Mass M1 = s.mass(0), M2 = s.mass(1);
   const int zi = G.zr, nb = T.nb;
   double P1 = pk(T, zi, G.k1), P2 = pk(T, zi, G.k2), P3 = pk(T, zi, G.k3);
   // G_gg(k1, k2, m2)
-  Occ a1 = occ1<P>(a, M2), a2 = occ2<P>(a, M2);
-  double nss = nsatnsat<P>(a, a1.ns, a2.ns, M2);
-  double u_k1_m2_f1 = P::u(make_nfw(T, M2, G.zt, a.h1.f), G.k1);
-  double u_k2_m2_f2 = P::u(make_nfw(T, M2, G.zt, a.h2.f), G.k2);
-  double u_k1_m1_f1 = P::u(make_nfw(T, M1, G.zt, a.h1.f), G.k1);
-  double u_k2_m1_f2 = P::u(make_nfw(T, M1, G.zt, a.h2.f), G.k2);
+  Occ a1 = s.occ1(1), a2 = s.occ2(1);
+  double nss = s.nss(1);
+  double u_k1_m2_f1 = s.template u<0>();
+  double u_k2_m2_f2 = s.template u<1>();
+  double u_k1_m1_f1 = s.template u<2>();
+  double u_k2_m1_f2 = s.template u<3>();
   double Ggg = a1.nc * a2.ns * u_k2_m2_f2 + a2.nc * a1.ns * u_k1_m2_f1 +
                nss * u_k1_m2_f1 * u_k2_m2_f2;
-  Occ b1 = occ1<P>(a, M1), b2 = occ2<P>(a, M1);
+  Occ b1 = s.occ1(0), b2 = s.occ2(0);
   double G_k1_m1_h1 = b1.nc + b1.ns * u_k1_m1_f1;
   double G_k2_m2_h2 = a2.nc + a2.ns * u_k2_m2_f2;
   double G_k2_m1_h2 = b2.nc + b2.ns * u_k2_m1_f2;
   double G_k1_m2_h1 = a1.nc + a1.ns * u_k1_m2_f1;
-  double u_k3_m1 = P::u(make_nfw(T, M1, G.zt, 1), G.k3);
-  double u_k3_m2 = P::u(make_nfw(T, M2, G.zt, 1), G.k3);
+  double u_k3_m1 = s.template u<4>();
+  double u_k3_m2 = s.template u<5>();
   double m1 = M1.m, m2 = M2.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
   double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * nb + M1.ir] *
              T.hmf[zi * nb + M2.ir] * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              (m1 * u_k3_m1 * Ggg * P3 + m2 * u_k3_m2 * G_k1_m1_h1 * G_k2_m2_h2 * P1 +
               m2 * u_k3_m2 * G_k2_m1_h2 * G_k1_m2_h1 * P2);
   double b = 1. / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z) * B;
-  return l1 * l2 * m1 * m2 * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) *
-         P::ap(a.t3 * G.l3) * b;
+  return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
 }
 
-// NNMap 3-halo, x = (log10 l1, log10 l2, phi, log10 m1..m3, z)
-// (src/NNMap_Model.cu:1003-1037).
-template <class P>
-G3L_HD double nnmap_3h(const DevTables &T, const TermArgs &a,
-                                           const double *x) {
-  Geo G = make_geo<P>(T, x, 7);
-  Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt),
-       M3 = make_mass(T, x[5], G.zt);
+// NNMap 3-halo (src/NNMap_Model.cu:1003-1037).
+template <class P, class S>
+G3L_HD double nnmap_3h(const DevTables &T, const TermArgs &a, const S &s) {
+  const Geo &G = s.geo();
+  const Mass M1 = s.mass(0), M2 = s.mass(1), M3 = s.mass(2);
   const int zi = G.zr, nb = T.nb;
-  double u3 = P::u(make_nfw(T, M3, G.zt, 1), G.k3);
-  Occ o1 = occ1<P>(a, M1);
-  double G1 = o1.nc + o1.ns * P::u(make_nfw(T, M1, G.zt, a.h1.f), G.k1);
-  Occ o2 = occ2<P>(a, M2);
-  double G2 = o2.nc + o2.ns * P::u(make_nfw(T, M2, G.zt, a.h2.f), G.k2);
+  double u3 = s.template u<0>();
+  Occ o1 = s.occ1(0);
+  double G1 = o1.nc + o1.ns * s.template u<1>();
+  Occ o2 = s.occ2(1);
+  double G2 = o2.nc + o2.ns * s.template u<2>();
   double m1 = M1.m, m2 = M2.m, m3 = M3.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
   double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * nb + M1.ir] *
-             T.hmf[zi * nb + M2.ir] * T.hmf[zi * nb + M3.ir] * m3 * u3 * G1 * G2 *
-             bilin(T, G.k1, G.k2, G.cphi, z) * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
-             T.bh[zi * nb + M3.ir];
+             T.hmf[zi * nb + M2.ir] * T.hmf[zi * nb + M3.ir] * m3 * u3 * G1 * G2 * s.bl() *
+             T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] * T.bh[zi * nb + M3.ir];
   double b = B / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z);
-  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) *
-         P::ap(a.t3 * G.l3) * b;
+  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
 }
 
 // NMapMap 1-halo (src/NMapMap_Model.cu:263-301).
-template <class P>
-G3L_HD double nmapmap_1h(const DevTables &T, const TermArgs &a,
-                                             const double *x) {
-  Geo G = make_geo<P>(T, x, 5);
-  Mass M = make_mass(T, x[3], G.zt);
+template <class P, class S>
+G3L_HD double nmapmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
+  const Geo &G = s.geo();
+  const Mass M = s.mass(0);
   const int zi = G.zr;
   double nbar = a.nb1[zi];
-  Occ o = occ1<P>(a, M);
+  Occ o = s.occ1(0);
   Nfw n1 = make_nfw(T, M, G.zt, 1);
-  double uf = P::u(make_nfw(T, M, G.zt, a.h1.f), G.k1);
+  double uf = s.template u<0>();
   double rho = T.rho[zi], m = M.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
   double B = 1. / nbar / rho / rho * T.hmf[zi * T.nb + M.ir] * m * m * (o.nc + o.ns * uf) *
-             P::u(n1, G.k2) * P::u(n1, G.k3);
+             s.template u<1>() * P::u(n1, G.k3);
   double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
-  return l1 * l2 * m * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) * P::ap(a.t3 * G.l3) * b /
-         nbar;
+  return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
 }
 
 // NMapMap 2-halo (src/NMapMap_Model.cu:402-468).
-template <class P>
-G3L_HD double nmapmap_2h(const DevTables &T, const TermArgs &a,
-                                             const double *x) {
-  Geo G = make_geo<P>(T, x, 6);
-  Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt);
+template <class P, class S>
+G3L_HD double nmapmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
+  const Geo &G = s.geo();
+  const Mass M1 = s.mass(0), M2 = s.mass(1);
   const int zi = G.zr, nb = T.nb;
   double nbar = a.nb1[zi];
   double P1 = pk(T, zi, G.k1), P2 = pk(T, zi, G.k2), P3 = pk(T, zi, G.k3);
-  Occ o1 = occ1<P>(a, M1), o2 = occ1<P>(a, M2);
-  double f = a.h1.f;
-  Nfw n1 = make_nfw(T, M1, G.zt, 1), n2 = make_nfw(T, M2, G.zt, 1);
-  double u_k2_m1 = P::u(n1, G.k2), u_k3_m1 = P::u(n1, G.k3);
-  double u_k2_m2 = P::u(n2, G.k2), u_k3_m2 = P::u(n2, G.k3);
-  double u_k1_m2_f = P::u(make_nfw(T, M2, G.zt, f), G.k1);
-  double u_k1_m1_f = P::u(make_nfw(T, M1, G.zt, f), G.k1);
+  Occ o1 = s.occ1(0), o2 = s.occ1(1);
+  double u_k2_m1 = s.template u<0>(), u_k3_m1 = s.template u<1>();
+  double u_k2_m2 = s.template u<2>(), u_k3_m2 = s.template u<3>();
+  double u_k1_m2_f = s.template u<4>();
+  double u_k1_m1_f = s.template u<5>();
   double rho = T.rho[zi], m1 = M1.m, m2 = M2.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
   double B = 1. / nbar / rho / rho * T.hmf[zi * nb + M1.ir] * T.hmf[zi * nb + M2.ir] *
              T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
@@ -636,32 +736,46 @@ G3L_HD double nmapmap_2h(const DevTables &T, const TermArgs &a,
               m1 * m2 * u_k3_m1 * u_k2_m2 * (o2.nc + o2.ns * u_k1_m2_f) * P3 +
               m2 * m2 * u_k2_m1 * u_k3_m2 * (o1.nc + o1.ns * u_k1_m1_f) * P1);
   double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
-  return l1 * l2 * m1 * m2 * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) *
-         P::ap(a.t3 * G.l3) * b / nbar;
+  return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
 }
 
 // NMapMap 3-halo: Ncen, Nsat at m3 with u_NFW(k1, m2, f) (src/NMapMap_Model.cu:593-632).
-template <class P>
-G3L_HD double nmapmap_3h(const DevTables &T, const TermArgs &a,
-                                             const double *x) {
-  Geo G = make_geo<P>(T, x, 7);
-  Mass M1 = make_mass(T, x[3], G.zt), M2 = make_mass(T, x[4], G.zt),
-       M3 = make_mass(T, x[5], G.zt);
+template <class P, class S>
+G3L_HD double nmapmap_3h(const DevTables &T, const TermArgs &a, const S &s) {
+  const Geo &G = s.geo();
+  const Mass M1 = s.mass(0), M2 = s.mass(1), M3 = s.mass(2);
   const int zi = G.zr, nb = T.nb;
   double nbar = a.nb1[zi];
-  Occ o = occ1<P>(a, M3);
-  double u_k2_m1 = P::u(make_nfw(T, M1, G.zt, 1), G.k2);
-  double u_k3_m2 = P::u(make_nfw(T, M2, G.zt, 1), G.k3);
-  double u_k1_m2_f = P::u(make_nfw(T, M2, G.zt, a.h1.f), G.k1);
+  Occ o = s.occ1(2);
+  double u_k2_m1 = s.template u<0>();
+  double u_k3_m2 = s.template u<1>();
+  double u_k1_m2_f = s.template u<2>();
   double rho = T.rho[zi], m1 = M1.m, m2 = M2.m, m3 = M3.m, w_ = G.w, z = G.z;
   double l1 = G.l1, l2 = G.l2;
   double B = 1. / nbar / rho / rho * T.hmf[zi * nb + M1.ir] * T.hmf[zi * nb + M2.ir] *
              T.hmf[zi * nb + M3.ir] * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              T.bh[zi * nb + M3.ir] * m1 * m2 * u_k2_m1 * u_k3_m2 * (o.nc + o.ns * u_k1_m2_f) *
-             bilin(T, G.k1, G.k2, G.cphi, z);
+             s.bl();
   double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
-  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * P::ap(a.t1 * l1) * P::ap(a.t2 * l2) *
-         P::ap(a.t3 * G.l3) * b / nbar;
+  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
+}
+
+// The integrand of kind KIND at one point (direct evaluation).
+template <int KIND, class P, class S>
+G3L_HD double integrand_src(const DevTables &T, const TermArgs &a, const S &s) {
+  if constexpr (KIND == kNN1) {
+    return nnmap_1h<P>(T, a, s);
+  } else if constexpr (KIND == kNN2) {
+    return nnmap_2h<P>(T, a, s);
+  } else if constexpr (KIND == kNN3) {
+    return nnmap_3h<P>(T, a, s);
+  } else if constexpr (KIND == kNM1) {
+    return nmapmap_1h<P>(T, a, s);
+  } else if constexpr (KIND == kNM2) {
+    return nmapmap_2h<P>(T, a, s);
+  } else {
+    return nmapmap_3h<P>(T, a, s);
+  }
 }
 
 // n_bar integrand, x = log10 m, theta slot 1 = z (src/generalFunctions.cu:233-244).

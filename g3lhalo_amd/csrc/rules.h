@@ -46,7 +46,7 @@ __device__ __forceinline__ void pair_of(int D, int p, int &i, int &j) {
 // Coordinates of point q of a region with centre c and half-width h.
 template <int D>
 __device__ __forceinline__ void rule_point(int q, const double *c, const double *h, double *x) {
-  if (D == 1) {
+  if constexpr (D == 1) {
     double w;
     if (q == 0) {
       x[0] = c[0];
