@@ -34,6 +34,13 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
               5: "NMapMap 2h", 6: "NMapMap 3h"}
 
 
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs of this workload scaled by a copy kernel of known size
+# (profiles/k_eval_fp64_nnmap550_counters.md): 569.4 MB read + 32.5 MB written
+# per k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
+TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 601983474}
+
+
 def workload(name: str, hodset: int = 0):
     reqs = []
     if name == "nnmap550":
@@ -81,6 +88,20 @@ def cpu_baseline(tables, reqs, seconds_budget: float):
             "seconds": round(t_used, 2)}, vals
 
 
+def copy_probe(mib: int) -> int:
+    """One elementwise float32 kernel that reads mib MiB and writes mib MiB: the
+    known-size reference against which the HBM traffic counters are scaled."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.add(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4}))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +112,12 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run an elementwise kernel that reads and writes this many MiB "
+                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args(argv)
+    if a.copy_probe_mib:
+        return copy_probe(a.copy_probe_mib)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the integrals are independent and shard by request "
               "with no exchange; run it with --gpus 1", file=sys.stderr)
@@ -204,7 +230,8 @@ def main(argv=None):
                        "note": "batched form: every integral of a step completes with the step"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": round(peak, 2), "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "frac": round(achieved / peak, 4),
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((a.workload, dom)),
                      "kernel": f"k_eval<{dom},P64> ({KIND_NAMES[dom]}, fused points + "
                                "integrand + rule)",
                      "avg_launch_ms": round(float(kind_ms[dom] / max(kind_launches[dom], 1)), 4),

@@ -12,10 +12,7 @@
 #include <new>
 #include <vector>
 
-#include "../../include/g3l.h"
-#include "kernels.h"
-#include "layout.h"
-#include "physics.h"
+#include "engine.h"
 
 using namespace g3l;
 
@@ -28,111 +25,17 @@ const double kC = 2.99792e5;
 const double kGMsun = 1.327e11;
 const double kMpcKm = 3.086e19;
 
-#define G3L_HIP(expr)                                           \
-  do {                                                          \
-    hipError_t e_ = (expr);                                     \
-    if (e_ != hipSuccess) {                                     \
-      std::fprintf(stderr, "g3l: %s failed: %s (%s:%d)\n", #expr, \
-                   hipGetErrorString(e_), __FILE__, __LINE__);  \
-      return G3L_EHIP;                                          \
-    }                                                           \
-  } while (0)
-
-// Device buffer that only grows.
-struct DevBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  int ensure(size_t want) {
-    if (want <= bytes) return G3L_OK;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    if (hipMalloc(&p, want) != hipSuccess) {
-      p = nullptr;
-      return G3L_ENOMEM;
-    }
-    bytes = want;
-    return G3L_OK;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <class T>
-  T *as() const {
-    return reinterpret_cast<T *>(p);
-  }
-};
-
-// Pinned host buffer that only grows.
-struct HostBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  int ensure(size_t want) {
-    if (want <= bytes) return G3L_OK;
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    bytes = 0;
-    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
-      p = nullptr;
-      return G3L_ENOMEM;
-    }
-    bytes = want;
-    return G3L_OK;
-  }
-  void release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <class T>
-  T *as() const {
-    return reinterpret_cast<T *>(p);
-  }
-};
-
-struct HodSet {
-  g3l_hod h1, h2;
-  double A, eps;
-  int identical;
-};
+DevHod to_dev(const g3l_hod &h) { return DevHod{h.f, h.alpha, h.mth, h.sigma, h.mprime, h.beta}; }
 
 }  // namespace
 
-struct g3l_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int nb = 0;
-  double zmin, zmax, kmin, kmax, mmin, mmax;
-  double H0, Om, OL, rho_crit;
-  double zmin_i, zmax_i, mmin_i, mmax_i;
-  std::vector<double> rho;
-  DevBuf tables;     // g p1 p2 w dwdz rho | hmf plin(+nb) bh conc
-  DevTables T{};
-  std::vector<HodSet> sets;
-  DevBuf nbar;       // [set][2][nb]
-  size_t nbar_sets_cap = 0;
-  // batched engine
-  DevBuf pool, split, lists, jobs, state, args, boxes, counters;
-  HostBuf hcounters;
-  // plain engine
-  DevBuf pgeo, psplit, plist, pjob, parg;
-  HostBuf hgeo, hsplit;
-  uint32_t pcap = 0;
-  hipEvent_t ev[4]{};
-  hipEvent_t kev[2 * kNumKinds]{};
-  g3l_batch_stats last{};
-};
-
-namespace {
+namespace g3l {
 
 int kind_of(int stat, int term) { return stat == G3L_NNMAP ? term : 3 + term; }
 
-DevHod to_dev(const g3l_hod &h) { return DevHod{h.f, h.alpha, h.mth, h.sigma, h.mprime, h.beta}; }
-
 const double *nbar_ptr(const g3l_ctx *c, int set, int which) {
-  return c->nbar.as<double>() + ((size_t)set * 2 + which) * c->nb;
+  return c->nbar_chunks[set / kSetsPerChunk].as<double>() +
+         ((size_t)(set % kSetsPerChunk) * 2 + which) * c->nb;
 }
 
 // Log-space HOD constants: log(mmin/M_th), log(mmin/M'), log(mmin).
@@ -223,6 +126,10 @@ uint32_t slot_cap(uint64_t maxeval, int pts, uint32_t unlimited_cap) {
   uint64_t c = 2 + (maxeval + (uint64_t)pts) / (2ull * (uint64_t)pts);
   return (uint32_t)std::min<uint64_t>(c, 1ull << 26);
 }
+
+}  // namespace g3l
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // Batched engine: all jobs refine together; per round one selection launch
@@ -562,19 +469,12 @@ int plain_cubature(g3l_ctx *c, int kind, const double *lo, const double *hi, uin
 // device in one batched run.
 int compute_nbar(g3l_ctx *c, int first, int count) {
   const int nb = c->nb;
-  size_t need_sets = (size_t)first + count;
-  if (need_sets > c->nbar_sets_cap) {
-    size_t cap = std::max<size_t>(16, c->nbar_sets_cap);
-    while (cap < need_sets) cap *= 2;
-    DevBuf nb2;
-    int rc = nb2.ensure(cap * 2 * nb * sizeof(double));
+  const size_t need_chunks = ((size_t)first + count + kSetsPerChunk - 1) / kSetsPerChunk;
+  while (c->nbar_chunks.size() < need_chunks) {
+    DevBuf b;
+    int rc = b.ensure((size_t)kSetsPerChunk * 2 * nb * sizeof(double));
     if (rc) return rc;
-    if (c->nbar.p)
-      G3L_HIP(hipMemcpy(nb2.p, c->nbar.p, (size_t)first * 2 * nb * sizeof(double),
-                        hipMemcpyDeviceToDevice));
-    c->nbar.release();
-    c->nbar = nb2;
-    c->nbar_sets_cap = cap;
+    c->nbar_chunks.push_back(b);
   }
   // run in chunks of sets to bound the pool
   const int chunk = 64;
@@ -615,8 +515,13 @@ int compute_nbar(g3l_ctx *c, int first, int count) {
       if (st[j].flags & (kJobPoolFull | kJobNonfinite)) return status_of(st[j].flags);
       nbv[j] = kLn10 * st[j].val;
     }
-    G3L_HIP(hipMemcpy(c->nbar.as<double>() + (size_t)s0 * 2 * nb, nbv.data(),
-                      nbv.size() * sizeof(double), hipMemcpyHostToDevice));
+    // copy run by run: sets s0..s1 may straddle two storage chunks
+    for (int s = s0; s < s1;) {
+      int e = std::min(s1, (s / kSetsPerChunk + 1) * kSetsPerChunk);
+      G3L_HIP(hipMemcpy(const_cast<double *>(nbar_ptr(c, s, 0)), nbv.data() + (size_t)(s - s0) * 2 * nb,
+                        (size_t)(e - s) * 2 * nb * sizeof(double), hipMemcpyHostToDevice));
+      s = e;
+    }
   }
   return G3L_OK;
 }
@@ -720,7 +625,8 @@ int g3l_ctx_destroy(g3l_ctx *c) {
   if (!c) return G3L_EINVAL;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->tables.release();
-  c->nbar.release();
+  for (auto &b : c->nbar_chunks) b.release();
+  c->nbar_chunks.clear();
   c->pool.release(); c->split.release(); c->lists.release(); c->jobs.release();
   c->state.release(); c->args.release(); c->boxes.release(); c->counters.release();
   c->hcounters.release();

@@ -1,0 +1,131 @@
+// engine.h -- internal declarations shared by the engine and the serving
+// loop: device/host buffers, the context, and the request helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <vector>
+
+#include "../../include/g3l.h"
+#include "kernels.h"
+#include "layout.h"
+#include "physics.h"
+
+#define G3L_HIP(expr)                                           \
+  do {                                                          \
+    hipError_t e_ = (expr);                                     \
+    if (e_ != hipSuccess) {                                     \
+      std::fprintf(stderr, "g3l: %s failed: %s (%s:%d)\n", #expr, \
+                   hipGetErrorString(e_), __FILE__, __LINE__);  \
+      return G3L_EHIP;                                          \
+    }                                                           \
+  } while (0)
+
+namespace g3l {
+
+// Device buffer that only grows.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return G3L_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipMalloc(&p, want) != hipSuccess) {
+      p = nullptr;
+      return G3L_ENOMEM;
+    }
+    bytes = want;
+    return G3L_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+// Pinned host buffer that only grows.
+struct HostBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want) {
+    if (want <= bytes) return G3L_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return G3L_ENOMEM;
+    }
+    bytes = want;
+    return G3L_OK;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const {
+    return reinterpret_cast<T *>(p);
+  }
+};
+
+struct HodSet {
+  g3l_hod h1, h2;
+  double A, eps;
+  int identical;
+};
+
+}  // namespace g3l
+
+struct g3l_ctx {
+  using DevBuf = g3l::DevBuf;
+  using HostBuf = g3l::HostBuf;
+  using HodSet = g3l::HodSet;
+  static constexpr int kNumKinds = g3l::kNumKinds;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nb = 0;
+  double zmin, zmax, kmin, kmax, mmin, mmax;
+  double H0, Om, OL, rho_crit;
+  double zmin_i, zmax_i, mmin_i, mmax_i;
+  std::vector<double> rho;
+  DevBuf tables;     // g p1 p2 w dwdz rho | hmf plin(+nb) bh conc
+  g3l::DevTables T{};
+  std::vector<HodSet> sets;
+  // n_bar per HOD set, [set][2][nb], in chunks of kSetsPerChunk sets that
+  // never move: integrals in flight hold pointers into them.
+  std::vector<DevBuf> nbar_chunks;
+  // batched engine
+  DevBuf pool, split, lists, jobs, state, args, boxes, counters;
+  HostBuf hcounters;
+  // plain engine
+  DevBuf pgeo, psplit, plist, pjob, parg;
+  HostBuf hgeo, hsplit;
+  uint32_t pcap = 0;
+  hipEvent_t ev[4]{};
+  hipEvent_t kev[2 * kNumKinds]{};
+  g3l_batch_stats last{};
+};
+
+namespace g3l {
+
+constexpr int kSetsPerChunk = 64;
+
+int kind_of(int stat, int term);
+const double *nbar_ptr(const g3l_ctx *c, int set, int which);
+TermArgs make_args(const g3l_ctx *c, const g3l_request &r);
+int term_box(const g3l_ctx *c, int term, double *lo, double *hi);
+double prefactor(const g3l_ctx *c, int stat, int term, double r);
+int check_request(const g3l_ctx *c, const g3l_request &r);
+int status_of(uint32_t flags);
+uint32_t slot_cap(uint64_t maxeval, int pts, uint32_t unlimited_cap);
+
+}  // namespace g3l
