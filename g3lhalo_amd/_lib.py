@@ -55,6 +55,16 @@ class BatchStats(C.Structure):
                 ("kind_launches", C.c_uint64 * 8), ("kind_evals", C.c_uint64 * 8)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("pool_slots", C.c_uint64), ("max_integrals", C.c_uint32),
+                ("precision", C.c_int32)]
+
+
+class ServerStatus(C.Structure):
+    _fields_ = [("in_flight", C.c_uint32), ("pending", C.c_uint32), ("completed", C.c_uint32),
+                ("admitted", C.c_uint32), ("rounds", C.c_uint64)]
+
+
 # Every symbol include/g3l.h declares, with its ctypes signature.
 SIGNATURES = {
     "g3l_version": (C.c_char_p, []),
@@ -73,6 +83,13 @@ SIGNATURES = {
     "g3l_integrand": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int32, C.c_size_t, _dp,
                                 _dp]),
     "g3l_last_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "g3l_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "g3l_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_size_t,
+                                    C.POINTER(Result), C.c_uint64]),
+    "g3l_server_step": (C.c_int, [C.c_void_p, C.POINTER(ServerStatus)]),
+    "g3l_server_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_size_t,
+                                  C.POINTER(C.c_size_t)]),
+    "g3l_server_destroy": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None
@@ -207,3 +224,46 @@ class Context:
         s = BatchStats()
         check("g3l_last_batch_stats", load().g3l_last_batch_stats(self.h, C.byref(s)))
         return s
+
+
+class Server:
+    """A g3l_server on a Context: submit requests, step, poll."""
+
+    def __init__(self, ctx: "Context", pool_slots=0, max_integrals=0, precision=FP64):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        cfg = ServerConfig(int(pool_slots), int(max_integrals), int(precision))
+        check("g3l_server_create", load().g3l_server_create(ctx.h, C.byref(cfg),
+                                                             C.byref(self.h)))
+        self._out = {}   # tag -> (requests, results) kept alive until polled
+
+    def submit(self, reqs, tag: int):
+        n = len(reqs)
+        arr = (Request * max(n, 1))(*reqs)
+        out = (Result * max(n, 1))()
+        self._out[int(tag)] = (arr, out, n)
+        rc = load().g3l_server_submit(self.h, arr, n, out, int(tag))
+        if rc != G3L_OK:
+            self._out.pop(int(tag), None)
+        check("g3l_server_submit", rc)
+
+    def step(self) -> ServerStatus:
+        st = ServerStatus()
+        check("g3l_server_step", load().g3l_server_step(self.h, C.byref(st)))
+        return st
+
+    def poll(self, cap=1024):
+        """Completed requests: list of (tag, [Result])."""
+        tags = (C.c_uint64 * cap)()
+        n = C.c_size_t()
+        check("g3l_server_poll", load().g3l_server_poll(self.h, tags, cap, C.byref(n)))
+        done = []
+        for i in range(n.value):
+            arr, out, k = self._out.pop(int(tags[i]))
+            done.append((int(tags[i]), [out[j] for j in range(k)]))
+        return done
+
+    def close(self):
+        if self.h:
+            load().g3l_server_destroy(self.h)
+            self.h = C.c_void_p()

@@ -40,14 +40,45 @@ struct SelectLaunch {
   uint8_t *split;
   uint64_t stride;
   EvalItem *lists[kNumKinds];
-  uint32_t *counters;   // per-kind list fill, then [kNumKinds] = active jobs
+  uint32_t *counters;   // per-kind list fill, [kNumKinds] = jobs still refining,
+                        // [kNumKinds + 1] = entries written to done
   uint32_t list_cap[kNumKinds];
+  const uint32_t *active;  // block b selects for job active[b]; null: job b
+  uint32_t *done;          // receives each job that finishes this round; may be null
+};
+
+// New integrals joining a running engine: their descriptors and parameters
+// go to table rows ids[t], their root regions to the evaluation lists.
+struct AdmitLaunch {
+  const uint32_t *ids;
+  const JobDesc *desc;
+  const TermArgs *args;
+  const double *boxes;  // per new job: lo[kMaxDim], hi[kMaxDim]
+  JobDesc *jobs;
+  TermArgs *args_table;
+  JobState *state;
+  double *geo;
+  uint8_t *split;
+  uint64_t stride;
+  EvalItem *lists[kNumKinds];
+  uint32_t *counters;
+  uint32_t n;
+};
+
+// Copies the states of the jobs named in ids into out (n entries).
+struct GatherLaunch {
+  const uint32_t *ids;
+  const JobState *state;
+  JobState *out;
+  uint32_t n;
 };
 
 hipError_t launch_eval(int kind, int prec, const EvalLaunch &L, hipStream_t s);
 hipError_t launch_roots(const RootsLaunch &L, hipStream_t s);
 hipError_t launch_select(const SelectLaunch &L, uint32_t njobs, hipStream_t s);
 hipError_t launch_zero(uint32_t *p, int n, hipStream_t s);
+hipError_t launch_admit(const AdmitLaunch &L, hipStream_t s);
+hipError_t launch_gather(const GatherLaunch &L, hipStream_t s);
 int eval_group_size(int kind);
 hipError_t launch_points(int kind, int prec, const DevTables &T, const TermArgs *args,
                          const double *x, uint32_t npts, double *f, hipStream_t s);

@@ -473,6 +473,43 @@ __global__ void k_roots(RootsLaunch L) {
   L.lists[d.kind][at] = EvalItem{d.base, j};
 }
 
+__global__ void k_admit(AdmitLaunch L) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L.n) return;
+  const uint32_t j = L.ids[t];
+  const JobDesc d = L.desc[t];
+  L.jobs[j] = d;
+  L.args_table[j] = L.args[t];
+  const double *lo = L.boxes + (size_t)t * 2 * kMaxDim;
+  const double *hi = lo + kMaxDim;
+  const uint64_t S = L.stride;
+  double h[kMaxDim];
+  for (int i = 0; i < d.dim; ++i) {
+    double c = 0.5 * (lo[i] + hi[i]);
+    h[i] = 0.5 * (hi[i] - lo[i]);
+    L.geo[(kFieldC + i) * S + d.base] = c;
+    L.geo[(kFieldH + i) * S + d.base] = h[i];
+  }
+  L.geo[kFieldVol * S + d.base] = box_volume(d.dim, h);
+  L.split[d.base] = 0;
+  JobState st;
+  st.neval = (uint64_t)d.pts;
+  st.n = 1;
+  st.flags = 0;
+  st.rounds = 0;
+  st.pad = 0;
+  st.val = 0;
+  st.err = 0;
+  L.state[j] = st;
+  uint32_t at = atomicAdd(&L.counters[d.kind], 1u);
+  L.lists[d.kind][at] = EvalItem{d.base, j};
+}
+
+__global__ void k_gather(GatherLaunch L) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < L.n) L.out[t] = L.state[L.ids[t]];
+}
+
 __global__ void k_zero(uint32_t *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -502,7 +539,7 @@ __device__ __forceinline__ uint64_t err_key(double e) {
 }
 
 __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
-  const int J = blockIdx.x;
+  const uint32_t J = L.active ? L.active[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
   __shared__ JobState s_st;
   __shared__ double s_red[256];
@@ -588,7 +625,10 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
   }
   __syncthreads();
   if (s_flag) {
-    if (tid == 0) L.state[J] = s_st;
+    if (tid == 0) {
+      L.state[J] = s_st;
+      if (L.done) L.done[atomicAdd(&L.counters[kNumKinds + 1], 1u)] = J;
+    }
     return;
   }
 
@@ -705,7 +745,10 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
   }
   __syncthreads();
   if (s_flag) {
-    if (tid == 0) L.state[J] = s_st;
+    if (tid == 0) {
+      L.state[J] = s_st;
+      if (L.done) L.done[atomicAdd(&L.counters[kNumKinds + 1], 1u)] = J;
+    }
     return;
   }
 
@@ -858,6 +901,18 @@ hipError_t launch_roots(const RootsLaunch &L, hipStream_t s) {
 hipError_t launch_select(const SelectLaunch &L, uint32_t njobs, hipStream_t s) {
   if (njobs == 0) return hipSuccess;
   hipLaunchKernelGGL(k_select, dim3(njobs), dim3(256), 0, s, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_admit(const AdmitLaunch &L, hipStream_t s) {
+  if (L.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_admit, dim3((L.n + 255) / 256), dim3(256), 0, s, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const GatherLaunch &L, hipStream_t s) {
+  if (L.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather, dim3((L.n + 255) / 256), dim3(256), 0, s, L);
   return hipGetLastError();
 }
 

@@ -151,6 +151,49 @@ int g3l_last_batch_stats(g3l_ctx *ctx, g3l_batch_stats *out);
 /* Version string of this build. */
 const char *g3l_version(void);
 
+/* ---------------------------------------------------------------------
+ * Serving: continuous batching.  No counterpart in the reference, whose
+ * callers (the drivers, and the fit loop of
+ * src/Fitstuff - not fixed/fitHelpers.cu:194-271) issue one
+ * NNMap_Model::calculateAll after another.  A server keeps a region pool
+ * and a job table in HBM; requests (groups of integrals, each a g3l_request
+ * as for g3l_eval_batch) may be submitted at any time, and each
+ * g3l_server_step admits waiting integrals in arrival order as pool memory
+ * and job rows free up, runs one refinement round over every integral in
+ * flight, and retires the finished ones.  An integral's result does not
+ * depend on what else is in flight: it equals g3l_eval_batch's for it.
+ * One server per context at a time; calls on one server are not
+ * thread-safe; HOD sets referenced by waiting integrals must not be
+ * cleared. */
+typedef struct g3l_server g3l_server;
+
+typedef struct g3l_server_config {
+  uint64_t pool_slots;     /* region slots shared by the integrals in flight,
+                              249 bytes of HBM each; 0 = 2^24 */
+  uint32_t max_integrals;  /* integrals in flight at most; 0 = 65536 */
+  int32_t precision;       /* g3l_precision */
+} g3l_server_config;
+
+typedef struct g3l_server_status {
+  uint32_t in_flight;   /* integrals refining */
+  uint32_t pending;     /* integrals waiting for admission */
+  uint32_t completed;   /* finished requests not yet polled */
+  uint32_t admitted;    /* integrals admitted by this step */
+  uint64_t rounds;      /* refinement rounds run so far */
+} g3l_server_status;
+
+int g3l_server_create(g3l_ctx *ctx, const g3l_server_config *cfg, g3l_server **out);
+/* Queues n integrals as one request; out[0..n) receives their results, in
+ * place, by the step that completes the request (out must stay valid until
+ * then).  EINVAL for an integral whose slot need exceeds the pool. */
+int g3l_server_submit(g3l_server *srv, const g3l_request *reqs, size_t n, g3l_result *out,
+                      uint64_t tag);
+/* One admission + refinement round; status may be NULL. */
+int g3l_server_step(g3l_server *srv, g3l_server_status *status);
+/* Tags of completed requests, in completion order (at most cap). */
+int g3l_server_poll(g3l_server *srv, uint64_t *tags, size_t cap, size_t *n);
+int g3l_server_destroy(g3l_server *srv);
+
 #ifdef __cplusplus
 }
 #endif
