@@ -121,7 +121,8 @@ struct Occ {
 
 // ----------------------------------------------------------------- Si, Ci
 // Rational approximations of src/generalFunctions.cu:44-87.
-G3L_HD void sici_small(double x, double &si, double &ci) {
+// Si(x) and the rational part of Ci(x) = gamma + ln x + cr for x < 4.
+G3L_HD void sici_small_parts(double x, double &si, double &cr) {
   double x2 = x * x;
   double x4 = x2 * x2;
   double x6 = x2 * x4;
@@ -144,7 +145,13 @@ G3L_HD void sici_small(double x, double &si, double &ci) {
       2.55533277086129636e-7 * x6 + 6.97071295760958946e-10 * x8 +
       1.38536352772778619e-12 * x10 + 1.89106054713059759e-15 * x12 +
       1.39759616731376855e-18 * x14;
-  ci = 0.5772156649 + log(x) + x2 * a / b;
+  cr = x2 * a / b;
+}
+
+G3L_HD void sici_small(double x, double &si, double &ci) {
+  double cr;
+  sici_small_parts(x, si, cr);
+  ci = 0.5772156649 + log(x) + cr;
 }
 
 // Auxiliary functions f(x), g(x) of the x >= 4 branch as rational functions
@@ -251,30 +258,45 @@ struct P64 {
     return exp(eps * (M.L + lmmin));
   }
 
-  // Truncated-NFW transform (src/generalFunctions.cu:6-30).  When both
-  // arguments are in the asymptotic branch (arg1 >= 4) the reference's sum
-  // equals g(arg1) + sin(arg1 c)(f(arg2) - 1/arg2) - g(arg2) cos(arg1 c)
-  // exactly; that form needs one sin/cos instead of three and has none of
-  // the O(1/arg) cancellation.
+  // Truncated-NFW transform (src/generalFunctions.cu:6-30),
+  //   u F = sin a1 (Si a2 - Si a1) + cos a1 (Ci a2 - Ci a1) - sin(a1 c) / a2,
+  // a1 = k r200 / c, a2 = a1 (1 + c), F = ln(1+c) - c/(1+c), rearranged
+  // (equal in exact arithmetic) by the branch of the Si/Ci approximation
+  // each argument falls in:
+  //   a1 >= 4:       u F = g(a1) + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c);
+  //   a1 < 4 <= a2:  u F = pi/2 sin a1 - sin a1 Si(a1) - cos a1 Ci(a1)
+  //                        + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c);
+  //   a2 < 4:        u = 1 + [(cos a1 - 1) ln(1+c) + (a1 c - sin a1 c) / a2
+  //                        + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si a2 - Si a1)] / F,
+  // with Ci = gamma + ln x + Cr below 4.  Every branch uses sin/cos of a1 c
+  // (a2 - a1) instead of a2, none has the reference's O(1) cancellations,
+  // and the pieces two branches share are computed once, so a wave whose
+  // lanes fall in different branches runs each piece at most once.
   G3L_HD static double u(const Nfw &n, double k) {
-    double arg1 = k * n.r200 / n.c;
-    double arg2 = arg1 * (1 + n.c);
-    if (arg1 >= 4) {
-      double g1, fm1, g2, fm2, sx, cx;
-      sici_aux_gfm(arg1, g1, fm1);
-      sici_aux_gfm(arg2, g2, fm2);
-      sincos(arg1 * n.c, &sx, &cx);
+    const double a1 = k * n.r200 / n.c;
+    const double a2 = a1 * (1 + n.c);
+    double sx, cx;
+    sincos(a1 * n.c, &sx, &cx);
+    double s1 = 0, c1 = 0, si1 = 0, cr1 = 0, g2 = 0, fm2 = 0;
+    if (a1 < 4) {
+      sincos(a1, &s1, &c1);
+      sici_small_parts(a1, si1, cr1);
+    }
+    if (a2 >= 4) sici_aux_gfm(a2, g2, fm2);
+    if (a1 >= 4) {
+      double g1, fm1;
+      sici_aux_gfm(a1, g1, fm1);
       return (g1 + sx * fm2 - g2 * cx) / n.F;
     }
-    double s1, c1;
-    sincos(arg1, &s1, &c1);
-    double si1, ci1, si2, ci2;
-    sici_sc(arg1, s1, c1, si1, ci1);
-    sici(arg2, si2, ci2);
-    double term1 = s1 * (si2 - si1);
-    double term2 = c1 * (ci2 - ci1);
-    double term3 = -sin(arg1 * n.c) / arg2;
-    return (term1 + term2 + term3) / n.F;
+    if (a2 >= 4) {
+      const double ci1 = 0.5772156649 + log(a1) + cr1;
+      return (0.5 * 3.14159265358979323846 * s1 - s1 * si1 - c1 * ci1 + sx * fm2 - g2 * cx) / n.F;
+    }
+    double si2, cr2;
+    sici_small_parts(a2, si2, cr2);
+    const double rest =
+        (c1 - 1) * n.lc + (a1 * n.c - sx) / a2 + c1 * (cr2 - cr1) + s1 * (si2 - si1);
+    return 1 + rest / n.F;
   }
 
   // U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42).
