@@ -320,8 +320,15 @@ def main(argv=None):
         line["latency_ms"]["served_p50"] = line["served"]["p50_ms"]
         line["latency_ms"]["served_p99"] = line["served"]["p99_ms"]
     if not a.no_cpu:
-        cb, _ = cpu_baseline(tables, reqs, a.cpu_seconds)
+        cb, ovals = cpu_baseline(tables, reqs, a.cpu_seconds)
         line["cpu_baseline"] = cb
+        # the same integrals from the FP64 batched step against the oracle
+        d = np.array([abs(results[i].value - v) / max(abs(v), 1e-300) for i, v in ovals.items()])
+        line["parity_vs_oracle"] = {
+            "integrals": len(d), "max_rel": float(d.max()), "median_rel": float(np.median(d)),
+            "frac_above_1e-4": float(np.mean(d > 1e-4)),
+            "note": "the CPU baseline's sample, full maxEval: batched FP64 values vs the "
+                    "oracle's for the same integrals"}
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0

@@ -182,13 +182,17 @@ struct Cache {
   static_assert(ProjTab<D, kL1>::n == 7 && ProjTab<D, kL2>::n == 7 &&
                     ProjTab<D, bit(2)>::n == 7 && ProjTab<D, bit(D - 1)>::n == 7,
                 "seven coordinates per dimension");
+  static_assert(ProjTab<D, kL1 | bit(D - 1)>::n == 17 && ProjTab<D, kL2 | bit(D - 1)>::n == 17,
+                "seventeen (l, z) classes");
   static_assert(ProjTab<D, bit(S::mdim[0]) | bit(D - 1)>::n == 17 &&
                     ProjTab<D, bit(S::mdim[1]) | bit(D - 1)>::n == 17 &&
                     ProjTab<D, bit(S::mdim[2]) | bit(D - 1)>::n == 17,
                 "seventeen (mass, z) classes");
   double l1[G][7], ap1[G][7], l2[G][7], ap2[G][7], cphi[G][7], z[G][7], w[G][7];
+  double zpre[G][7], zfac[G][7];
   int zr[G][7], zt[G][7];
   MassC m[S::NM][G][17];
+  double pk[S::has_pk ? 2 : 1][G][17];  // P(k1), P(k2) per (l, z) class
   double u[NU_TOTAL];
   double bl[G * NB];
 };
@@ -253,79 +257,120 @@ struct CachedSrc {
   __device__ __forceinline__ double bl() const {
     return C.bl[g * Cache<KIND>::NB + kProj<D, bilin_mask<D>()>.cls[q]];
   }
+  __device__ __forceinline__ double pk1() const {
+    return C.pk[0][g][kProj<D, kL1 | bit(D - 1)>.cls[q]];
+  }
+  __device__ __forceinline__ double pk2() const {
+    return C.pk[S::has_pk ? 1 : 0][g][kProj<D, kL2 | bit(D - 1)>.cls[q]];
+  }
+  __device__ __forceinline__ double zpre() const {
+    return C.zpre[g][kProj<D, bit(D - 1)>.cls[q]];
+  }
+  __device__ __forceinline__ double zfac() const {
+    return C.zfac[g][kProj<D, bit(D - 1)>.cls[q]];
+  }
 };
 
-// Pass 1 item i: which region and class, per wave-aligned segment.
+// Pass 1 item layout: wave-aligned segments of items that run the same code:
+// l1 and l2 with their filters (2 x G x 7), cos phi (G x 7), the redshift
+// look-ups and z-only factors (G x 7), every mass coordinate's (m, z) class
+// (NM x G x 17), and for the 2-halo terms P(k1), P(k2) per (l, z) class
+// (2 x G x 17).
+template <int KIND>
+struct P1Layout {
+  using S = Spec<KIND>;
+  static constexpr int G = KGroup<KIND>::G;
+  static constexpr int pad(int n) { return (n + 63) / 64 * 64; }
+  static constexpr int L = 0, PHI = pad(2 * G * 7), Z = PHI + pad(G * 7), MZ = Z + pad(G * 7);
+  static constexpr int PK = MZ + pad(S::NM * G * 17);
+  static constexpr int END = PK + (S::has_pk ? pad(2 * G * 17) : 0);
+};
+
 template <int KIND, class P>
 __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<KIND>::D, KGroup<KIND>::G> &St,
                                       Cache<KIND> &C, int i, int ng) {
   using S = Spec<KIND>;
+  using Lay = P1Layout<KIND>;
   constexpr int D = S::D, G = KGroup<KIND>::G;
-  constexpr int S7 = (G * 7 + 63) / 64 * 64, S17 = (G * 17 + 63) / 64 * 64;
-  int seg, r;
-  if (i < 4 * S7) {
-    seg = i / S7;
-    r = i - seg * S7;
-    if (r >= G * 7) return;
-    const int g = r / 7, c = r - g * 7;
+  double x[D];
+  if (i < Lay::PHI) {
+    const int r = i - Lay::L;
+    if (r >= 2 * G * 7) return;
+    const int which = r / (G * 7), rr = r - which * G * 7, g = rr / 7, c = rr - g * 7;
     if (g >= ng || St.slot[g] == 0xffffffffu) return;
     const TermArgs &a = St.args[g];
-    double x[D];
-    if (seg == 0) {
+    if (which == 0) {
       rule_point<D>(kProj<D, kL1>.rep[c], St.c[g], St.h[g], x);
       const double l = P::l(x[0]);
       C.l1[g][c] = l;
       C.ap1[g][c] = P::ap(a.t1 * l);
-    } else if (seg == 1) {
+    } else {
       rule_point<D>(kProj<D, kL2>.rep[c], St.c[g], St.h[g], x);
       const double l = P::l(x[1]);
       C.l2[g][c] = l;
       C.ap2[g][c] = P::ap(a.t2 * l);
-    } else if (seg == 2) {
-      rule_point<D>(kProj<D, bit(2)>.rep[c], St.c[g], St.h[g], x);
-      C.cphi[g][c] = cos(x[2]);
-    } else {
-      rule_point<D>(kProj<D, bit(D - 1)>.rep[c], St.c[g], St.h[g], x);
-      const double z = x[D - 1];
-      const int zr = zix_round(T, z);
-      C.z[g][c] = z;
-      C.zr[g][c] = zr;
-      C.zt[g][c] = zix_trunc(T, z);
-      C.w[g][c] = T.w[zr];
     }
-    return;
-  }
-  r = i - 4 * S7;
-  const int slot = r / S17;
-  if (slot >= S::NM) return;
-  r -= slot * S17;
-  if (r >= G * 17) return;
-  const int g = r / 17, c = r - g * 17;
-  if (g >= ng || St.slot[g] == 0xffffffffu) return;
-  const TermArgs &a = St.args[g];
-  const int md = slot == 0 ? S::mdim[0] : slot == 1 ? S::mdim[1] : S::mdim[2];
-  int rep;
-  if (slot == 0) rep = kProj<D, bit(S::mdim[0]) | bit(D - 1)>.rep[c];
-  else if (slot == 1) rep = kProj<D, bit(S::mdim[S::NM > 1 ? 1 : 0]) | bit(D - 1)>.rep[c];
-  else rep = kProj<D, bit(S::mdim[S::NM > 2 ? 2 : 0]) | bit(D - 1)>.rep[c];
-  double x[D];
-  rule_point<D>(rep, St.c[g], St.h[g], x);
-  const int zt = zix_trunc(T, x[D - 1]);
-  const Mass M = make_mass(T, x[md], zt);
-  MassC &e = C.m[slot][g][c];
-  e.m = M.m;
-  e.L = M.L;
-  e.r200 = M.r200;
-  e.ir = M.ir;
-  e.it = M.it;
-  const Occ o1 = occ1<P>(a, M);
-  e.nc1 = o1.nc;
-  e.ns1 = o1.ns;
-  if constexpr (S::two_hods) {
-    const Occ o2 = occ2<P>(a, M);
-    e.nc2 = o2.nc;
-    e.ns2 = o2.ns;
-    if (slot == S::nss_slot) e.nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+  } else if (i < Lay::Z) {
+    const int r = i - Lay::PHI;
+    if (r >= G * 7) return;
+    const int g = r / 7, c = r - g * 7;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return;
+    rule_point<D>(kProj<D, bit(2)>.rep[c], St.c[g], St.h[g], x);
+    C.cphi[g][c] = cos(x[2]);
+  } else if (i < Lay::MZ) {
+    const int r = i - Lay::Z;
+    if (r >= G * 7) return;
+    const int g = r / 7, c = r - g * 7;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return;
+    rule_point<D>(kProj<D, bit(D - 1)>.rep[c], St.c[g], St.h[g], x);
+    const double z = x[D - 1];
+    const int zr = zix_round(T, z);
+    const double w = T.w[zr];
+    C.z[g][c] = z;
+    C.zr[g][c] = zr;
+    C.zt[g][c] = zix_trunc(T, z);
+    C.w[g][c] = w;
+    zfactors<KIND>(T, St.args[g], zr, z, w, C.zpre[g][c], C.zfac[g][c]);
+  } else if (i < Lay::PK) {
+    const int r = i - Lay::MZ;
+    if (r >= S::NM * G * 17) return;
+    const int slot = r / (G * 17), rr = r - slot * G * 17, g = rr / 17, c = rr - g * 17;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return;
+    const TermArgs &a = St.args[g];
+    const int md = slot == 0 ? S::mdim[0] : slot == 1 ? S::mdim[1] : S::mdim[2];
+    int rep;
+    if (slot == 0) rep = kProj<D, bit(S::mdim[0]) | bit(D - 1)>.rep[c];
+    else if (slot == 1) rep = kProj<D, bit(S::mdim[S::NM > 1 ? 1 : 0]) | bit(D - 1)>.rep[c];
+    else rep = kProj<D, bit(S::mdim[S::NM > 2 ? 2 : 0]) | bit(D - 1)>.rep[c];
+    rule_point<D>(rep, St.c[g], St.h[g], x);
+    const int zt = zix_trunc(T, x[D - 1]);
+    const Mass M = make_mass(T, x[md], zt);
+    MassC &e = C.m[slot][g][c];
+    e.m = M.m;
+    e.L = M.L;
+    e.r200 = M.r200;
+    e.ir = M.ir;
+    e.it = M.it;
+    const Occ o1 = occ1<P>(a, M);
+    e.nc1 = o1.nc;
+    e.ns1 = o1.ns;
+    if constexpr (S::two_hods) {
+      const Occ o2 = occ2<P>(a, M);
+      e.nc2 = o2.nc;
+      e.ns2 = o2.ns;
+      if (slot == S::nss_slot) e.nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+    }
+  } else if constexpr (S::has_pk) {
+    const int r = i - Lay::PK;
+    if (r >= 2 * G * 17) return;
+    const int which = r / (G * 17), rr = r - which * G * 17, g = rr / 17, c = rr - g * 17;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return;
+    // P(k) at k = l / w, the same expressions as the per-point Geo
+    rule_point<D>(which == 0 ? kProj<D, kL1 | bit(D - 1)>.rep[c] : kProj<D, kL2 | bit(D - 1)>.rep[c],
+                  St.c[g], St.h[g], x);
+    const double l = P::l(x[which]);
+    const int zr = zix_round(T, x[D - 1]);
+    C.pk[which][g][c] = pk(T, zr, l / T.w[zr]);
   }
 }
 
@@ -408,8 +453,7 @@ __global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
   constexpr int NP = KGroup<KIND>::P;
   constexpr int G = KGroup<KIND>::G;
   constexpr int NT = KGroup<KIND>::NT;
-  constexpr int S7 = (G * 7 + 63) / 64 * 64, S17 = (G * 17 + 63) / 64 * 64;
-  static_assert(4 * S7 + S::NM * S17 <= NT, "pass 1 fits one item per thread");
+  static_assert(P1Layout<KIND>::END <= NT, "pass 1 fits one item per thread");
   constexpr int P2 = (Cache<KIND>::NU_TOTAL + 63) / 64 * 64 +
                      (S::has_bilin ? G * Cache<KIND>::NB : 0);
   static_assert(P2 <= 2 * NT, "pass 2 fits two items per thread");

@@ -558,14 +558,14 @@ struct Spec;
 template <>
 struct Spec<kNN1> {
   static constexpr int D = 5, NM = 1, NU = 2, nss_slot = 0;
-  static constexpr bool two_hods = true, has_bilin = false;
+  static constexpr bool two_hods = true, has_bilin = false, has_pk = false;
   static constexpr int mdim[3] = {3, 0, 0};
   static constexpr USpec u[6] = {{kL1 | bit(3) | bit(4), 1, 0, 1}, {kL2 | bit(3) | bit(4), 2, 0, 2}};
 };
 template <>
 struct Spec<kNN2> {
   static constexpr int D = 6, NM = 2, NU = 6, nss_slot = 1;
-  static constexpr bool two_hods = true, has_bilin = false;
+  static constexpr bool two_hods = true, has_bilin = false, has_pk = true;
   static constexpr int mdim[3] = {3, 4, 0};
   static constexpr USpec u[6] = {{kL1 | bit(4) | bit(5), 1, 1, 1},   // u(k1, m2, f1)
                                  {kL2 | bit(4) | bit(5), 2, 1, 2},   // u(k2, m2, f2)
@@ -577,7 +577,7 @@ struct Spec<kNN2> {
 template <>
 struct Spec<kNN3> {
   static constexpr int D = 7, NM = 3, NU = 3, nss_slot = -1;
-  static constexpr bool two_hods = true, has_bilin = true;
+  static constexpr bool two_hods = true, has_bilin = true, has_pk = false;
   static constexpr int mdim[3] = {3, 4, 5};
   static constexpr USpec u[6] = {{kL3 | bit(5) | bit(6), 3, 2, 0},   // u(k3, m3)
                                  {kL1 | bit(3) | bit(6), 1, 0, 1},   // u(k1, m1, f1)
@@ -586,7 +586,7 @@ struct Spec<kNN3> {
 template <>
 struct Spec<kNM1> {
   static constexpr int D = 5, NM = 1, NU = 2, nss_slot = -1;
-  static constexpr bool two_hods = false, has_bilin = false;
+  static constexpr bool two_hods = false, has_bilin = false, has_pk = false;
   static constexpr int mdim[3] = {3, 0, 0};
   static constexpr USpec u[6] = {{kL1 | bit(3) | bit(4), 1, 0, 1},   // u(k1, m, f)
                                  {kL2 | bit(3) | bit(4), 2, 0, 0}};  // u(k2, m)
@@ -594,7 +594,7 @@ struct Spec<kNM1> {
 template <>
 struct Spec<kNM2> {
   static constexpr int D = 6, NM = 2, NU = 6, nss_slot = -1;
-  static constexpr bool two_hods = false, has_bilin = false;
+  static constexpr bool two_hods = false, has_bilin = false, has_pk = true;
   static constexpr int mdim[3] = {3, 4, 0};
   static constexpr USpec u[6] = {{kL2 | bit(3) | bit(5), 2, 0, 0},   // u(k2, m1)
                                  {kL3 | bit(3) | bit(5), 3, 0, 0},   // u(k3, m1)
@@ -606,7 +606,7 @@ struct Spec<kNM2> {
 template <>
 struct Spec<kNM3> {
   static constexpr int D = 7, NM = 3, NU = 3, nss_slot = -1;
-  static constexpr bool two_hods = false, has_bilin = true;
+  static constexpr bool two_hods = false, has_bilin = true, has_pk = false;
   static constexpr int mdim[3] = {3, 4, 5};
   static constexpr USpec u[6] = {{kL2 | bit(3) | bit(6), 2, 0, 0},   // u(k2, m1)
                                  {kL3 | bit(4) | bit(6), 3, 1, 0},   // u(k3, m2)
@@ -618,6 +618,23 @@ constexpr unsigned bilin_mask() { return kL3 | bit(D - 1); }
 
 G3L_HD double f_of(const TermArgs &a, int fsel) {
   return fsel == 0 ? 1.0 : fsel == 1 ? a.h1.f : a.h2.f;
+}
+
+// Redshift-only factors of the integrands, per redshift index zi:
+//   zpre = 1/n1/n2/rho (NNMap), 1/n/rho/rho (NMapMap), the leading factor of B;
+//   zfac = g p1 p2 (1+z) / dwdz / w^3 (NNMap), g^2 p1 (1+z)^2 / dwdz / w^2
+//   (NMapMap), the projection factor b = B zfac.
+template <int KIND>
+G3L_HD void zfactors(const DevTables &T, const TermArgs &a, int zi, double z, double w,
+                     double &zpre, double &zfac) {
+  if constexpr (KIND <= kNN3) {
+    zpre = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi];
+    zfac = 1. / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w / w / w * (1. + z);
+  } else {
+    const double rho = T.rho[zi];
+    zpre = 1. / a.nb1[zi] / rho / rho;
+    zfac = 1. / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w / w * (1. + z) * (1. + z);
+  }
 }
 
 // Direct source: every quantity computed at the point itself.
@@ -647,6 +664,18 @@ struct DirectSrc {
   G3L_HD double ap1() const { return P::ap(a.t1 * G.l1); }
   G3L_HD double ap2() const { return P::ap(a.t2 * G.l2); }
   G3L_HD double bl() const { return bilin(T, G.k1, G.k2, G.cphi, G.z); }
+  G3L_HD double pk1() const { return pk(T, G.zr, G.k1); }
+  G3L_HD double pk2() const { return pk(T, G.zr, G.k2); }
+  G3L_HD double zpre() const {
+    double p, f;
+    zfactors<KIND>(T, a, G.zr, G.z, G.w, p, f);
+    return p;
+  }
+  G3L_HD double zfac() const {
+    double p, f;
+    zfactors<KIND>(T, a, G.zr, G.z, G.w, p, f);
+    return f;
+  }
 };
 
 // The six integrands over a value source S (DirectSrc, or the kernel's
@@ -663,10 +692,10 @@ G3L_HD double nnmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   double u3 = P::u(make_nfw(T, M, G.zt, 1), G.k3);
   double u1 = s.template u<0>();
   double u2 = s.template u<1>();
-  double m = M.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
-  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * T.nb + M.ir] * m * u3;
+  double m = M.m, l1 = G.l1, l2 = G.l2;
+  double B = s.zpre() * T.hmf[zi * T.nb + M.ir] * m * u3;
   B *= o1.nc * o2.ns * u2 + o2.nc * o1.ns * u1 + nss * u1 * u2;
-  double b = B / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z);
+  double b = B * s.zfac();
   return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
 }
 
@@ -676,7 +705,7 @@ G3L_HD double nnmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
   const Geo &G = s.geo();
   const Mass M1 = s.mass(0), M2 = s.mass(1);
   const int zi = G.zr, nb = T.nb;
-  double P1 = pk(T, zi, G.k1), P2 = pk(T, zi, G.k2), P3 = pk(T, zi, G.k3);
+  double P1 = s.pk1(), P2 = s.pk2(), P3 = pk(T, zi, G.k3);
   // G_gg(k1, k2, m2)
   Occ a1 = s.occ1(1), a2 = s.occ2(1);
   double nss = s.nss(1);
@@ -693,12 +722,12 @@ G3L_HD double nnmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
   double G_k1_m2_h1 = a1.nc + a1.ns * u_k1_m2_f1;
   double u_k3_m1 = s.template u<4>();
   double u_k3_m2 = s.template u<5>();
-  double m1 = M1.m, m2 = M2.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
-  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * nb + M1.ir] *
+  double m1 = M1.m, m2 = M2.m, l1 = G.l1, l2 = G.l2;
+  double B = s.zpre() * T.hmf[zi * nb + M1.ir] *
              T.hmf[zi * nb + M2.ir] * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              (m1 * u_k3_m1 * Ggg * P3 + m2 * u_k3_m2 * G_k1_m1_h1 * G_k2_m2_h2 * P1 +
               m2 * u_k3_m2 * G_k2_m1_h2 * G_k1_m2_h1 * P2);
-  double b = 1. / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z) * B;
+  double b = s.zfac() * B;
   return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
 }
 
@@ -713,11 +742,11 @@ G3L_HD double nnmap_3h(const DevTables &T, const TermArgs &a, const S &s) {
   double G1 = o1.nc + o1.ns * s.template u<1>();
   Occ o2 = s.occ2(1);
   double G2 = o2.nc + o2.ns * s.template u<2>();
-  double m1 = M1.m, m2 = M2.m, m3 = M3.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
-  double B = 1. / a.nb1[zi] / a.nb2[zi] / T.rho[zi] * T.hmf[zi * nb + M1.ir] *
+  double m1 = M1.m, m2 = M2.m, m3 = M3.m, l1 = G.l1, l2 = G.l2;
+  double B = s.zpre() * T.hmf[zi * nb + M1.ir] *
              T.hmf[zi * nb + M2.ir] * T.hmf[zi * nb + M3.ir] * m3 * u3 * G1 * G2 * s.bl() *
              T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] * T.bh[zi * nb + M3.ir];
-  double b = B / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / w_ / w_ / w_ * (1. + z);
+  double b = B * s.zfac();
   return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
 }
 
@@ -731,10 +760,10 @@ G3L_HD double nmapmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   Occ o = s.occ1(0);
   Nfw n1 = make_nfw(T, M, G.zt, 1);
   double uf = s.template u<0>();
-  double rho = T.rho[zi], m = M.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
-  double B = 1. / nbar / rho / rho * T.hmf[zi * T.nb + M.ir] * m * m * (o.nc + o.ns * uf) *
+  double m = M.m, l1 = G.l1, l2 = G.l2;
+  double B = s.zpre() * T.hmf[zi * T.nb + M.ir] * m * m * (o.nc + o.ns * uf) *
              s.template u<1>() * P::u(n1, G.k3);
-  double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
+  double b = B * s.zfac();
   return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
 }
 
@@ -745,19 +774,19 @@ G3L_HD double nmapmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
   const Mass M1 = s.mass(0), M2 = s.mass(1);
   const int zi = G.zr, nb = T.nb;
   double nbar = a.nb1[zi];
-  double P1 = pk(T, zi, G.k1), P2 = pk(T, zi, G.k2), P3 = pk(T, zi, G.k3);
+  double P1 = s.pk1(), P2 = s.pk2(), P3 = pk(T, zi, G.k3);
   Occ o1 = s.occ1(0), o2 = s.occ1(1);
   double u_k2_m1 = s.template u<0>(), u_k3_m1 = s.template u<1>();
   double u_k2_m2 = s.template u<2>(), u_k3_m2 = s.template u<3>();
   double u_k1_m2_f = s.template u<4>();
   double u_k1_m1_f = s.template u<5>();
-  double rho = T.rho[zi], m1 = M1.m, m2 = M2.m, w_ = G.w, z = G.z, l1 = G.l1, l2 = G.l2;
-  double B = 1. / nbar / rho / rho * T.hmf[zi * nb + M1.ir] * T.hmf[zi * nb + M2.ir] *
+  double m1 = M1.m, m2 = M2.m, l1 = G.l1, l2 = G.l2;
+  double B = s.zpre() * T.hmf[zi * nb + M1.ir] * T.hmf[zi * nb + M2.ir] *
              T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              (m1 * m2 * u_k2_m1 * u_k3_m2 * (o2.nc + o2.ns * u_k1_m2_f) * P2 +
               m1 * m2 * u_k3_m1 * u_k2_m2 * (o2.nc + o2.ns * u_k1_m2_f) * P3 +
               m2 * m2 * u_k2_m1 * u_k3_m2 * (o1.nc + o1.ns * u_k1_m1_f) * P1);
-  double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
+  double b = B * s.zfac();
   return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
 }
 
@@ -772,13 +801,13 @@ G3L_HD double nmapmap_3h(const DevTables &T, const TermArgs &a, const S &s) {
   double u_k2_m1 = s.template u<0>();
   double u_k3_m2 = s.template u<1>();
   double u_k1_m2_f = s.template u<2>();
-  double rho = T.rho[zi], m1 = M1.m, m2 = M2.m, m3 = M3.m, w_ = G.w, z = G.z;
+  double m1 = M1.m, m2 = M2.m, m3 = M3.m;
   double l1 = G.l1, l2 = G.l2;
-  double B = 1. / nbar / rho / rho * T.hmf[zi * nb + M1.ir] * T.hmf[zi * nb + M2.ir] *
+  double B = s.zpre() * T.hmf[zi * nb + M1.ir] * T.hmf[zi * nb + M2.ir] *
              T.hmf[zi * nb + M3.ir] * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              T.bh[zi * nb + M3.ir] * m1 * m2 * u_k2_m1 * u_k3_m2 * (o.nc + o.ns * u_k1_m2_f) *
              s.bl();
-  double b = B / T.dwdz[zi] * T.g[zi] * T.g[zi] * T.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
+  double b = B * s.zfac();
   return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
 }
 
