@@ -36,9 +36,9 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this workload scaled by a copy kernel of known size
-# (profiles/k_eval_fp64_nnmap550_counters.md): 569.4 MB read + 32.5 MB written
-# per k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
-TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 601983474}
+# (profiles/k_eval_nnmap550_counters.md): 569.4 MB read + 32.5 MB written per
+# k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
+TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 601915309}
 
 
 def workload(name: str, hodset: int = 0):
@@ -315,6 +315,15 @@ def main(argv=None):
         "evals_per_step": int(sum(int(r.neval) for r in results)),
         "failed_integrals": len(bad),
     }
+    # device n_bar(z) for a sweep of HOD vectors (the set-up of a fit step):
+    # 2 x 128 GK15 integrals per HOD pair, all sets in one call
+    hods = T.hod_prior_draws(256, seed=20220405)
+    t0 = time.perf_counter()
+    ctx.hodsets(hods, hods)
+    nbar_ms = (time.perf_counter() - t0) * 1e3
+    line["nbar_sweep"] = {"hod_sets": 256, "integrals": 256 * 2 * tables.nbins,
+                          "ms": round(nbar_ms, 3),
+                          "integrals_per_s": round(256 * 2 * tables.nbins / (nbar_ms * 1e-3), 1)}
     if a.serve_requests > 0:
         line["served"] = serve(ctx, tables, a.serve_requests, a.serve_rate)
         line["latency_ms"]["served_p50"] = line["served"]["p50_ms"]

@@ -163,3 +163,27 @@ def test_mixed_batch_terms(ctx, oracle_model, theta):
         d = rel(r.value, o["value"])
         print(f"mixed term stat={s} term={t}: {r.value:.10e} oracle {o['value']:.10e} rel {d:.2e}")
         assert d < TOL
+
+
+def test_many_hodsets_densities_match_oracle(tables):
+    """Device n_bar(z) for a sweep of prior HODs made in one call (the HOD-sweep
+    workload's set-up), straddling the 64-set storage chunks."""
+    import oracle as O
+    from g3lhalo_amd import _lib
+    say("densities: 70 prior HOD sets in one call")
+    c = _lib.Context(tables)
+    try:
+        c.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)       # shifts the sweep off chunk boundaries
+        h1 = T.hod_prior_draws(70, seed=3)
+        h2 = T.hod_prior_draws(70, seed=5)
+        first = c.hodsets(h1, h2)
+        assert first == 1
+        for k in (0, 1, 30, 62, 63, 64, 69):
+            m = O.Model(tables)
+            m.set_hods(h1[k], h2[k])
+            r0, a0, b0 = m.densities()
+            r, a, b = c.densities(first + k)
+            assert np.array_equal(r, r0)
+            assert rel(a, a0).max() < 1e-12 and rel(b, b0).max() < 1e-12, k
+    finally:
+        c.close()
