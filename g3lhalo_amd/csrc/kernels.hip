@@ -157,6 +157,18 @@ struct MassC {
   double nc1, ns1, nc2, ns2, nss;
 };
 
+// u_NFW normalisation (c, ln(1+c), F) per f selector (1, f1, f2).
+struct NfwC {
+  double c[3], lc[3], F[3];
+};
+
+// Whether pass 1 caches the u_NFW normalisations per (mass, z) class (5-D and
+// 6-D kinds).  The 7-D kinds compute them per u_NFW item: their classes are
+// few, their pass 1 is the longest, and the larger cache costs them more than
+// the logarithms it saves.
+template <int KIND>
+constexpr bool cache_nfw() { return Spec<KIND>::D <= 6; }
+
 template <int KIND, int I>
 struct UF {
   static constexpr int D = Spec<KIND>::D;
@@ -192,6 +204,7 @@ struct Cache {
   double zpre[G][7], zfac[G][7];
   int zr[G][7], zt[G][7];
   MassC m[S::NM][G][17];
+  NfwC nf[cache_nfw<KIND>() ? S::NM : 1][cache_nfw<KIND>() ? G : 1][17];
   double pk[S::has_pk ? 2 : 1][G][17];  // P(k1), P(k2) per (l, z) class
   double u[NU_TOTAL];
   double bl[G * NB];
@@ -247,6 +260,14 @@ struct CachedSrc {
     return Occ{e.nc2, e.ns2};
   }
   __device__ __forceinline__ double nss(int i) const { return mc(i).nss; }
+  __device__ __forceinline__ Nfw nfw(int i, int fsel) const {
+    static_assert(cache_nfw<KIND>(), "per-point u_NFW normalisations come from the cache");
+    const int cls = i == 0   ? kProj<D, bit(S::mdim[0]) | bit(D - 1)>.cls[q]
+                    : i == 1 ? kProj<D, bit(S::mdim[S::NM > 1 ? 1 : 0]) | bit(D - 1)>.cls[q]
+                             : kProj<D, bit(S::mdim[S::NM > 2 ? 2 : 0]) | bit(D - 1)>.cls[q];
+    const NfwC &e = C.nf[i][g][cls];
+    return Nfw{mc(i).r200, e.c[fsel], e.lc[fsel], e.F[fsel]};
+  }
   template <int I>
   __device__ __forceinline__ double u() const {
     using F = UF<KIND, I>;
@@ -274,7 +295,8 @@ struct CachedSrc {
 // Pass 1 item layout: wave-aligned segments of items that run the same code:
 // l1 and l2 with their filters (2 x G x 7), cos phi (G x 7), the redshift
 // look-ups and z-only factors (G x 7), every mass coordinate's (m, z) class
-// (NM x G x 17), and for the 2-halo terms P(k1), P(k2) per (l, z) class
+// (NM x G x 17; the coordinates of one kind need the same u_NFW
+// normalisations), and for the 2-halo terms P(k1), P(k2) per (l, z) class
 // (2 x G x 17).
 template <int KIND>
 struct P1Layout {
@@ -360,6 +382,20 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<KIND>
       e.ns2 = o2.ns;
       if (slot == S::nss_slot) e.nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
     }
+    if constexpr (cache_nfw<KIND>()) {
+      NfwC &en = C.nf[slot][g][c];
+#pragma unroll
+      for (int fs = 0; fs < 3; ++fs) {
+        const bool need = slot == 0 ? uses_nfw<KIND>(0, fs)
+                                    : slot == 1 ? uses_nfw<KIND>(1, fs) : uses_nfw<KIND>(2, fs);
+        if (need) {
+          const Nfw nf = make_nfw(T, M, zt, f_of(a, fs));
+          en.c[fs] = nf.c;
+          en.lc[fs] = nf.lc;
+          en.F[fs] = nf.F;
+        }
+      }
+    }
   } else if constexpr (S::has_pk) {
     const int r = i - Lay::PK;
     if (r >= 2 * G * 17) return;
@@ -404,14 +440,20 @@ __device__ __forceinline__ bool u_inputs(const DevTables &T,
       l = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cp);
     }
     k = l / w;
-    const MassC &e = C.m[F::s.slot][g][kProj<D, bit(S::mdim[F::s.slot]) | bit(D - 1)>.cls[q]];
-    Mass M;
-    M.m = e.m;
-    M.L = e.L;
-    M.ir = e.ir;
-    M.it = e.it;
-    M.r200 = e.r200;
-    n = make_nfw(T, M, C.zt[g][cz], f_of(St.args[g], F::s.fsel));
+    const int cm = kProj<D, bit(S::mdim[F::s.slot]) | bit(D - 1)>.cls[q];
+    const MassC &e = C.m[F::s.slot][g][cm];
+    if constexpr (cache_nfw<KIND>()) {
+      const NfwC &en = C.nf[F::s.slot][g][cm];
+      n = Nfw{e.r200, en.c[F::s.fsel], en.lc[F::s.fsel], en.F[F::s.fsel]};
+    } else {
+      Mass M;
+      M.m = e.m;
+      M.L = e.L;
+      M.ir = e.ir;
+      M.it = e.it;
+      M.r200 = e.r200;
+      n = make_nfw(T, M, C.zt[g][cz], f_of(St.args[g], F::s.fsel));
+    }
     dst = i;
     return true;
   }

@@ -35,6 +35,23 @@
 
 namespace g3l {
 
+// a / b for the normal, finite operands of the u_NFW arithmetic: the hardware
+// reciprocal estimate refined by two Newton-Raphson steps, then one residual
+// correction of the quotient (within one unit in the last place; about a third
+// of the instructions of the IEEE division sequence, which also handles
+// denormal and infinite operands that cannot occur here).
+G3L_HD double fdiv(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+#else
+  return a / b;
+#endif
+}
+
 // Tables resident in HBM (z-major 2-D tables, nb*nb + nb for P_lin).
 struct DevTables {
   int nb;
@@ -137,7 +154,7 @@ G3L_HD void sici_small_parts(double x, double &si, double &cr) {
   double b = 1 + 1.01162145739225565e-2 * x2 + 4.99175116169755106e-5 * x4 +
              1.55654986308745614e-7 * x6 + 3.28067571055789734e-10 * x8 +
              4.5049097575386581e-13 * x10 + 3.21107051193712168e-16 * x12;
-  si = x * a / b;
+  si = fdiv(x * a, b);
   a = -0.25 + 7.51851524438898291e-3 * x2 - 1.27528342240267686e-4 * x4 +
       1.05297363846239184e-6 * x6 - 4.68889508144848019e-9 * x8 +
       1.06480802891189243e-11 * x10 - 9.93728488857585407e-15 * x12;
@@ -145,51 +162,13 @@ G3L_HD void sici_small_parts(double x, double &si, double &cr) {
       2.55533277086129636e-7 * x6 + 6.97071295760958946e-10 * x8 +
       1.38536352772778619e-12 * x10 + 1.89106054713059759e-15 * x12 +
       1.39759616731376855e-18 * x14;
-  cr = x2 * a / b;
-}
-
-G3L_HD void sici_small(double x, double &si, double &ci) {
-  double cr;
-  sici_small_parts(x, si, cr);
-  ci = 0.5772156649 + log(x) + cr;
-}
-
-// Auxiliary functions f(x), g(x) of the x >= 4 branch as rational functions
-// of t = 1/x^2 (the reference's coefficients, Horner's rule).
-G3L_HD void sici_aux(double x, double &f, double &g) {
-  double t = 1.0 / (x * x);
-  double fn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
-                  -4.94701168645415959931e11, 4.94816688199951963482e12),
-                  1.00795182980368574617e13), 4.20968180571076940208e12),
-                  6.40533830574022022911e11), 4.33736238870432522765e10),
-                  1.43073403821274636888e9), 2.37750310125431834034e7),
-                  1.96396372895146869801e5), 7.44437068161936700618e2);
-  double fd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
-                  1.11535493509914254097e13, 1.43468549171581016479e13),
-                  5.06084464593475076774e12), 7.08501308149515401563e11),
-                  4.58595115847765779830e10), 1.47478952192985464958e9),
-                  2.41535670165126845144e7), 1.97865247031583951450e5),
-                  7.46437068161927678031e2);
-  double gn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
-                  -1.36517137670871689e12, 6.43291613143049485e12),
-                  1.81004487464664575e13), 7.57664583257834349e12),
-                  1.09049528450362786e12), 6.83052205423625007e10),
-                  2.06297595146763354e9), 3.12557570795778731e7),
-                  2.35239181626478200e5), 8.1359520115168615e2);
-  double gd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
-                  3.99653257887490811e13, 4.01839087307656620e13),
-                  1.17164723371736605e13), 1.39866710696414565e12),
-                  7.87465017341829930e10), 2.23355543278099360e9),
-                  3.26026661647090822e7), 2.40036752835578777e5),
-                  8.19595201151451564e2);
-  f = (1 + t * fn) / ((1 + t * fd) * x);
-  g = (1 + t * gn) * t / (1 + t * gd);
+  cr = fdiv(x2 * a, b);
 }
 
 // g(x) and f(x) - 1/x for x >= 4, the latter from the exact coefficient
 // differences of f's numerator and denominator (no cancellation).
 G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
-  double t = 1.0 / (x * x);
+  double t = fdiv(1.0, x * x);
   double fd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
                   1.11535493509914254097e13, 1.43468549171581016479e13),
                   5.06084464593475076774e12), 7.08501308149515401563e11),
@@ -214,26 +193,8 @@ G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
                   7.87465017341829930e10), 2.23355543278099360e9),
                   3.26026661647090822e7), 2.40036752835578777e5),
                   8.19595201151451564e2);
-  g = (1 + t * gn) * t / (1 + t * gd);
-  fm = t * dn / ((1 + t * fd) * x);
-}
-
-// Si and Ci at x; sx, cx = sin x, cos x (used when x >= 4).
-G3L_HD void sici_sc(double x, double sx, double cx, double &si, double &ci) {
-  if (x < 4) {
-    sici_small(x, si, ci);
-  } else {
-    double f, g;
-    sici_aux(x, f, g);
-    si = 0.5 * 3.14159265358979323846 - f * cx - g * sx;
-    ci = f * sx - g * cx;
-  }
-}
-
-G3L_HD void sici(double x, double &si, double &ci) {
-  double sx = 0, cx = 0;
-  if (x >= 4) sincos(x, &sx, &cx);
-  sici_sc(x, sx, cx, si, ci);
+  g = fdiv((1 + t * gn) * t, 1 + t * gd);
+  fm = fdiv(t * dn, (1 + t * fd) * x);
 }
 
 // ======================================================================
@@ -273,7 +234,7 @@ struct P64 {
   // and the pieces two branches share are computed once, so a wave whose
   // lanes fall in different branches runs each piece at most once.
   G3L_HD static double u(const Nfw &n, double k) {
-    const double a1 = k * n.r200 / n.c;
+    const double a1 = fdiv(k * n.r200, n.c);
     const double a2 = a1 * (1 + n.c);
     double sx, cx;
     sincos(a1 * n.c, &sx, &cx);
@@ -286,17 +247,18 @@ struct P64 {
     if (a1 >= 4) {
       double g1, fm1;
       sici_aux_gfm(a1, g1, fm1);
-      return (g1 + sx * fm2 - g2 * cx) / n.F;
+      return fdiv(g1 + sx * fm2 - g2 * cx, n.F);
     }
     if (a2 >= 4) {
       const double ci1 = 0.5772156649 + log(a1) + cr1;
-      return (0.5 * 3.14159265358979323846 * s1 - s1 * si1 - c1 * ci1 + sx * fm2 - g2 * cx) / n.F;
+      return fdiv(0.5 * 3.14159265358979323846 * s1 - s1 * si1 - c1 * ci1 + sx * fm2 - g2 * cx,
+                  n.F);
     }
     double si2, cr2;
     sici_small_parts(a2, si2, cr2);
     const double rest =
-        (c1 - 1) * n.lc + (a1 * n.c - sx) / a2 + c1 * (cr2 - cr1) + s1 * (si2 - si1);
-    return 1 + rest / n.F;
+        (c1 - 1) * n.lc + fdiv(a1 * n.c - sx, a2) + c1 * (cr2 - cr1) + s1 * (si2 - si1);
+    return 1 + fdiv(rest, n.F);
   }
 
   // U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42).
@@ -612,6 +574,17 @@ struct Spec<kNM3> {
                                  {kL3 | bit(4) | bit(6), 3, 1, 0},   // u(k3, m2)
                                  {kL1 | bit(4) | bit(6), 1, 1, 1}};  // u(k1, m2, f)
 };
+// Whether the integrand of KIND needs u_NFW's normalisation (c, ln(1+c), F)
+// for mass slot `slot` with f selector `fsel`: its partial u_NFW factors, and
+// the 1-halo terms' per-point u_NFW(k3, m) with f = 1.
+template <int KIND>
+constexpr bool uses_nfw(int slot, int fsel) {
+  if ((KIND == kNN1 || KIND == kNM1) && slot == 0 && fsel == 0) return true;
+  for (int i = 0; i < Spec<KIND>::NU; ++i)
+    if (Spec<KIND>::u[i].slot == slot && Spec<KIND>::u[i].fsel == fsel) return true;
+  return false;
+}
+
 // Coordinates of the tree-level bispectrum: l1, l2, phi, z.
 template <int D>
 constexpr unsigned bilin_mask() { return kL3 | bit(D - 1); }
@@ -655,11 +628,12 @@ struct DirectSrc {
   G3L_HD Occ occ1(int i) const { return g3l::occ1<P>(a, M[i]); }
   G3L_HD Occ occ2(int i) const { return g3l::occ2<P>(a, M[i]); }
   G3L_HD double nss(int i) const { return nsatnsat<P>(a, occ1(i).ns, occ2(i).ns, M[i]); }
+  G3L_HD Nfw nfw(int slot, int fsel) const { return make_nfw(T, M[slot], G.zt, f_of(a, fsel)); }
   template <int I>
   G3L_HD double u() const {
     constexpr USpec s = S::u[I];
     const double k = s.kidx == 1 ? G.k1 : s.kidx == 2 ? G.k2 : G.k3;
-    return P::u(make_nfw(T, M[s.slot], G.zt, f_of(a, s.fsel)), k);
+    return P::u(nfw(s.slot, s.fsel), k);
   }
   G3L_HD double ap1() const { return P::ap(a.t1 * G.l1); }
   G3L_HD double ap2() const { return P::ap(a.t2 * G.l2); }
@@ -689,7 +663,7 @@ G3L_HD double nnmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   const int zi = G.zr;
   Occ o1 = s.occ1(0), o2 = s.occ2(0);
   double nss = s.nss(0);
-  double u3 = P::u(make_nfw(T, M, G.zt, 1), G.k3);
+  double u3 = P::u(s.nfw(0, 0), G.k3);
   double u1 = s.template u<0>();
   double u2 = s.template u<1>();
   double m = M.m, l1 = G.l1, l2 = G.l2;
@@ -758,7 +732,7 @@ G3L_HD double nmapmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   const int zi = G.zr;
   double nbar = a.nb1[zi];
   Occ o = s.occ1(0);
-  Nfw n1 = make_nfw(T, M, G.zt, 1);
+  Nfw n1 = s.nfw(0, 0);
   double uf = s.template u<0>();
   double m = M.m, l1 = G.l1, l2 = G.l2;
   double B = s.zpre() * T.hmf[zi * T.nb + M.ir] * m * m * (o.nc + o.ns * uf) *
