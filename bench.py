@@ -203,6 +203,11 @@ def main(argv=None):
     ctx = _lib.Context(tables)
     sid = ctx.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
     reqs, desc = workload(a.workload, sid)
+    # block shapes per integrand kind, timed on this device (results do not
+    # depend on them)
+    tuned = {"fp64": ctx.autotune(sid, _lib.FP64)}
+    if not a.fp64_only:
+        tuned["mixed"] = ctx.autotune(sid, _lib.MIXED)
 
     def run_form(precision):
         for _ in range(a.warmup):
@@ -312,6 +317,8 @@ def main(argv=None):
                                "integrand + rule)",
                      "avg_launch_ms": round(float(kind_ms[dom] / max(kind_launches[dom], 1)), 4),
                      "per_kernel": per_kind},
+        "autotune": {p: {KIND_NAMES[k]: r["group"][r["variant"]] for k, r in t.items()}
+                     for p, t in tuned.items()},
         "evals_per_step": int(sum(int(r.neval) for r in results)),
         "failed_integrals": len(bad),
     }

@@ -55,6 +55,11 @@ class BatchStats(C.Structure):
                 ("kind_launches", C.c_uint64 * 8), ("kind_evals", C.c_uint64 * 8)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("variant", C.c_int32 * 8), ("ms", (C.c_double * 4) * 8),
+                ("group", (C.c_int32 * 4) * 8)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("pool_slots", C.c_uint64), ("max_integrals", C.c_uint32),
                 ("precision", C.c_int32)]
@@ -83,6 +88,8 @@ SIGNATURES = {
     "g3l_integrand": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int32, C.c_size_t, _dp,
                                 _dp]),
     "g3l_last_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "g3l_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneReport)]),
+    "g3l_set_variant": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "g3l_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "g3l_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_size_t,
                                     C.POINTER(Result), C.c_uint64]),
@@ -219,6 +226,20 @@ class Context:
         check("g3l_integrand", load().g3l_integrand(self.h, C.byref(req), int(precision),
                                                      x.shape[0], _p(x), _p(f)))
         return f
+
+    def autotune(self, hodset=0, precision=FP64) -> dict:
+        """Times the launch variants and keeps the fastest per kind; returns
+        {kind: {"variant", "ms": [per variant], "group": [regions per block]}}."""
+        r = TuneReport()
+        check("g3l_autotune", load().g3l_autotune(self.h, int(hodset), int(precision),
+                                                   C.byref(r)))
+        nv = sum(1 for v in range(4) if r.group[1][v] > 0)
+        return {k: {"variant": int(r.variant[k]), "ms": [round(r.ms[k][v], 3) for v in range(nv)],
+                    "group": [int(r.group[k][v]) for v in range(nv)]} for k in range(1, 7)}
+
+    def set_variant(self, kind, precision, variant):
+        check("g3l_set_variant", load().g3l_set_variant(self.h, int(kind), int(precision),
+                                                         int(variant)))
 
     def last_stats(self) -> BatchStats:
         s = BatchStats()

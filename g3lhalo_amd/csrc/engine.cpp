@@ -232,7 +232,7 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
       EL.list = lists[k];
       EL.count = (uint32_t)std::min<uint64_t>(hcnt[k], per_kind[k]);
       if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k], s));
-      G3L_HIP(launch_eval(k, prec, EL, s));
+      G3L_HIP(launch_eval(k, prec, c->variant[prec == G3L_MIXED][k], EL, s));
       if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k + 1], s));
       ++launches;
       ++kind_launches[k];
@@ -392,7 +392,7 @@ int plain_eval_batch(g3l_ctx *c, int kind, int D, std::vector<HRegion> &R, size_
   EL.split = c->psplit.as<uint8_t>();
   EL.stride = cap;
   EL.njobs = 1;
-  G3L_HIP(launch_eval(kind, G3L_FP64, EL, s));
+  G3L_HIP(launch_eval(kind, G3L_FP64, c->variant[0][kind], EL, s));
   G3L_HIP(hipMemcpy2DAsync(hg + kFieldVal * cap, cap * sizeof(double),
                            c->pgeo.as<double>() + kFieldVal * cap, cap * sizeof(double),
                            nR * sizeof(double), 2, hipMemcpyDeviceToHost, s));
@@ -808,6 +808,94 @@ int g3l_integrand(g3l_ctx *c, const g3l_request *r, int32_t precision, size_t np
 int g3l_last_batch_stats(g3l_ctx *c, g3l_batch_stats *out) {
   if (!c || !out) return G3L_EINVAL;
   *out = c->last;
+  return G3L_OK;
+}
+
+int g3l_autotune(g3l_ctx *c, int32_t hodset, int32_t precision, g3l_tune_report *out) {
+  if (!c || hodset < 0 || hodset >= (int)c->sets.size()) return G3L_EINVAL;
+  if (precision != G3L_FP64 && precision != G3L_MIXED) return G3L_EINVAL;
+  // calibration: 55 aperture triplets x 1h/2h/3h for each statistic, at the
+  // production tolerance and budget (launch sizes like the real workloads)
+  const double arcmin = 2.909e-4;
+  std::vector<g3l_request> reqs[2];
+  for (int st = 0; st < 2; ++st)
+    for (int i = 0; i < 55; ++i) {
+      const int a = i % 10, b = (i * 3) % 10, d = (i * 7) % 10;
+      for (int term = 1; term <= 3; ++term) {
+        g3l_request r{};
+        r.stat = st;
+        r.term = term;
+        r.hodset = hodset;
+        r.type1 = 1;
+        r.type2 = 2;
+        r.theta[0] = 0.5 * std::pow(64.0, a / 9.0) * arcmin;
+        r.theta[1] = 0.5 * std::pow(64.0, b / 9.0) * arcmin;
+        r.theta[2] = 0.5 * std::pow(64.0, d / 9.0) * arcmin;
+        r.reltol = 1e-4;
+        r.maxeval = 5000000;
+        reqs[st].push_back(r);
+      }
+    }
+  const int pi = precision == G3L_MIXED;
+  const int nv = eval_num_variants();
+  double ms[kNumKinds][8] = {};
+  const g3l_batch_stats keep = c->last;
+  for (int v = 0; v < nv && v < 8; ++v) {
+    for (int k = 1; k < kNumKinds; ++k) c->variant[pi][k] = v;
+    for (int st = 0; st < 2; ++st) {
+      EngineJobs E;
+      for (const g3l_request &r : reqs[st]) {
+        const int kind = kind_of(r.stat, r.term);
+        JobDesc d{};
+        d.kind = kind;
+        d.dim = kind_dim(kind);
+        d.pts = rule_points(d.dim);
+        d.args = (int)E.args.size();
+        d.reltol = r.reltol;
+        d.maxeval = r.maxeval;
+        d.cap = slot_cap(d.maxeval, d.pts, 1u << 16);
+        E.desc.push_back(d);
+        E.args.push_back(make_args(c, r));
+        double box[2 * kMaxDim] = {0};
+        term_box(c, r.term, box, box + kMaxDim);
+        E.boxes.insert(E.boxes.end(), box, box + 2 * kMaxDim);
+      }
+      std::vector<JobState> jst;
+      for (int rep = 0; rep < 2; ++rep) {  // the first run warms up
+        int rc = run_engine(c, E, jst, true, precision);
+        if (rc) {
+          c->last = keep;
+          return rc;
+        }
+      }
+      for (int k = 1; k < kNumKinds; ++k) ms[k][v] += c->last.kind_ms[k];
+    }
+  }
+  c->last = keep;
+  for (int k = 1; k < kNumKinds; ++k) {
+    int best = 0;
+    for (int v = 1; v < nv; ++v)
+      if (ms[k][v] < ms[k][best]) best = v;
+    c->variant[pi][k] = best;
+  }
+  if (out) {
+    *out = g3l_tune_report{};
+    for (int k = 0; k < kNumKinds && k < 8; ++k) {
+      out->variant[k] = c->variant[pi][k];
+      for (int v = 0; v < nv && v < 4; ++v) {
+        out->ms[k][v] = ms[k][v];
+        out->group[k][v] = eval_group_size(k, v);
+      }
+    }
+  }
+  return G3L_OK;
+}
+
+int g3l_set_variant(g3l_ctx *c, int32_t kind, int32_t precision, int32_t variant) {
+  if (!c || kind < 1 || kind >= kNumKinds || variant < 0 || variant >= eval_num_variants())
+    return G3L_EINVAL;
+  if (precision != G3L_FP64 && precision != G3L_MIXED) return G3L_EINVAL;
+  c->variant[precision == G3L_MIXED][kind] = variant;
   return G3L_OK;
 }
 

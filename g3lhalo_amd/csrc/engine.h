@@ -110,6 +110,8 @@ struct g3l_ctx {
   DevBuf pgeo, psplit, plist, pjob, parg;
   HostBuf hgeo, hsplit;
   uint32_t pcap = 0;
+  // launch variant (block shape) per precision and integrand kind
+  int variant[2][kNumKinds] = {{0, 1, 2, 2, 1, 1, 2}, {0, 1, 2, 2, 1, 1, 2}};
   hipEvent_t ev[4]{};
   hipEvent_t kev[2 * kNumKinds]{};
   g3l_batch_stats last{};

@@ -73,13 +73,15 @@ struct GatherLaunch {
   uint32_t n;
 };
 
-hipError_t launch_eval(int kind, int prec, const EvalLaunch &L, hipStream_t s);
+// variant: block shape, 0 .. eval_num_variants() - 1 (the same values for all)
+hipError_t launch_eval(int kind, int prec, int variant, const EvalLaunch &L, hipStream_t s);
+int eval_num_variants();
 hipError_t launch_roots(const RootsLaunch &L, hipStream_t s);
 hipError_t launch_select(const SelectLaunch &L, uint32_t njobs, hipStream_t s);
 hipError_t launch_zero(uint32_t *p, int n, hipStream_t s);
 hipError_t launch_admit(const AdmitLaunch &L, hipStream_t s);
 hipError_t launch_gather(const GatherLaunch &L, hipStream_t s);
-int eval_group_size(int kind);
+int eval_group_size(int kind, int variant);
 hipError_t launch_points(int kind, int prec, const DevTables &T, const TermArgs *args,
                          const double *x, uint32_t npts, double *f, hipStream_t s);
 

@@ -33,21 +33,34 @@ __device__ __forceinline__ double integrand(const DevTables &T, const TermArgs &
   }
 }
 
+// Launch variants: a kernel template parameter KIND carries the integrand
+// kind in its low four bits and a block-shape variant above them; kind_of_kv
+// and variant_of_kv take it apart.  Every variant computes the same values.
+constexpr int kVariantShift = 4;
+constexpr int kNumVariants = 3;
+__host__ __device__ constexpr int kind_of_kv(int kv) { return kv & ((1 << kVariantShift) - 1); }
+__host__ __device__ constexpr int variant_of_kv(int kv) { return kv >> kVariantShift; }
+__host__ __device__ constexpr int make_kv(int kind, int v) { return kind | (v << kVariantShift); }
+
 // G regions per block and NT threads: every pass gives each thread one
 // item per round, in straight-line code -- no loop for the compiler to hoist
 // the FP64 constants of the special functions out of, which otherwise pins
 // them in registers for the whole kernel and drops occupancy to one wave per
-// SIMD.  G is chosen per kind so that the dominant pass fills its rounds:
-//   n_bar  (1-D): 68 regions x 15 points = 1020 of 1024 threads;
-//   1-halo (5-D): 11 x 93 = 1023 points; u_NFW pass 11 x 66 = 726;
-//   2-halo (6-D):  4 x 149 = 596 points of 640; u_NFW pass 4 x 318 = 1272
-//                  in two rounds of 640;
-//   3-halo (7-D):  4 x 241 = 964 points; u_NFW + bispectrum pass 636 + 228.
-template <int KIND>
+// SIMD.  Three shapes per dimension, picked per kind at run time
+// (g3l_autotune): variant 0 fills one block of up to 1024 threads, variants
+// 1 and 2 halve and quarter it so that more blocks share a compute unit and
+// one block's barriers overlap another's work:
+//   n_bar  (1-D): 68 regions x 15 points = 1020 threads (all variants);
+//   1-halo (5-D): 11 x 93 = 1023 points, 5 x 93 = 465 of 512, 2 x 93 = 186 of 192;
+//   2-halo (6-D):  4 x 149 = 596 points of 640 (u_NFW pass 4 x 318 = 1272 in
+//                  two rounds), 2 x 149 = 298 of 320, 1 x 149 of 192;
+//   3-halo (7-D):  4 x 241 = 964 points, 2 x 241 = 482 of 512, 1 x 241 of 256.
+template <int KV>
 struct KGroup {
-  static constexpr int D = kind_dim(KIND);
+  static constexpr int D = kind_dim(kind_of_kv(KV));
+  static constexpr int V = variant_of_kv(KV);
   static constexpr int P = rule_points(D);
-  static constexpr int G = D == 1 ? 68 : D == 5 ? 11 : 4;
+  static constexpr int G = D == 1 ? 68 : D == 5 ? (V == 2 ? 2 : V ? 5 : 11) : (V == 2 ? 1 : V ? 2 : 4);
   static constexpr int NT = (G * P + 63) / 64 * 64;
   static_assert(NT <= 1024, "one block holds every point of its regions");
 };
@@ -115,7 +128,7 @@ __device__ __forceinline__ void reduce_regions(const EvalLaunch &L, const Stage<
 // Direct evaluation (n_bar): every point evaluates its integrand in full.
 template <int KIND, class P>
 __global__ __launch_bounds__(1024) void k_eval_direct(EvalLaunch L) {
-  constexpr int D = kind_dim(KIND);
+  constexpr int D = kind_dim(kind_of_kv(KIND));
   constexpr int NP = KGroup<KIND>::P;
   constexpr int G = KGroup<KIND>::G;
   constexpr int NT = KGroup<KIND>::NT;
@@ -130,7 +143,7 @@ __global__ __launch_bounds__(1024) void k_eval_direct(EvalLaunch L) {
     if (S.slot[g] != 0xffffffffu) {
       double x[D];
       rule_point<D>(q, S.c[g], S.h[g], x);
-      s_f[tid] = integrand<KIND, P>(T, S.args[g], x);
+      s_f[tid] = integrand<kind_of_kv(KIND), P>(T, S.args[g], x);
     }
   }
   __syncthreads();
@@ -167,12 +180,12 @@ struct NfwC {
 // few, their pass 1 is the longest, and the larger cache costs them more than
 // the logarithms it saves.
 template <int KIND>
-constexpr bool cache_nfw() { return Spec<KIND>::D <= 6; }
+constexpr bool cache_nfw() { return Spec<kind_of_kv(KIND)>::D <= 6; }
 
 template <int KIND, int I>
 struct UF {
-  static constexpr int D = Spec<KIND>::D;
-  static constexpr USpec s = Spec<KIND>::u[I];
+  static constexpr int D = Spec<kind_of_kv(KIND)>::D;
+  static constexpr USpec s = Spec<kind_of_kv(KIND)>::u[I];
   static constexpr int n = ProjTab<D, s.mask>::n;
 };
 // Offset of u factor I in the flat cache, in units of regions x classes.
@@ -187,7 +200,7 @@ struct UOff<KIND, 0> {
 
 template <int KIND>
 struct Cache {
-  using S = Spec<KIND>;
+  using S = Spec<kind_of_kv(KIND)>;
   static constexpr int D = S::D, G = KGroup<KIND>::G;
   static constexpr int NU_TOTAL = UOff<KIND, S::NU>::v;
   static constexpr int NB = S::has_bilin ? ProjTab<D, bilin_mask<D>()>::n : 1;
@@ -213,7 +226,7 @@ struct Cache {
 // Per-point view of the cache (the value source of the integrands).
 template <int KIND, class P>
 struct CachedSrc {
-  using S = Spec<KIND>;
+  using S = Spec<kind_of_kv(KIND)>;
   static constexpr int D = S::D, G = KGroup<KIND>::G;
   const Cache<KIND> &C;
   const TermArgs &a;
@@ -300,7 +313,7 @@ struct CachedSrc {
 // (2 x G x 17).
 template <int KIND>
 struct P1Layout {
-  using S = Spec<KIND>;
+  using S = Spec<kind_of_kv(KIND)>;
   static constexpr int G = KGroup<KIND>::G;
   static constexpr int pad(int n) { return (n + 63) / 64 * 64; }
   static constexpr int L = 0, PHI = pad(2 * G * 7), Z = PHI + pad(G * 7), MZ = Z + pad(G * 7);
@@ -309,9 +322,9 @@ struct P1Layout {
 };
 
 template <int KIND, class P>
-__device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<KIND>::D, KGroup<KIND>::G> &St,
+__device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
                                       Cache<KIND> &C, int i, int ng) {
-  using S = Spec<KIND>;
+  using S = Spec<kind_of_kv(KIND)>;
   using Lay = P1Layout<KIND>;
   constexpr int D = S::D, G = KGroup<KIND>::G;
   double x[D];
@@ -352,7 +365,7 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<KIND>
     C.zr[g][c] = zr;
     C.zt[g][c] = zix_trunc(T, z);
     C.w[g][c] = w;
-    zfactors<KIND>(T, St.args[g], zr, z, w, C.zpre[g][c], C.zfac[g][c]);
+    zfactors<kind_of_kv(KIND)>(T, St.args[g], zr, z, w, C.zpre[g][c], C.zfac[g][c]);
   } else if (i < Lay::PK) {
     const int r = i - Lay::MZ;
     if (r >= S::NM * G * 17) return;
@@ -386,8 +399,8 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<KIND>
       NfwC &en = C.nf[slot][g][c];
 #pragma unroll
       for (int fs = 0; fs < 3; ++fs) {
-        const bool need = slot == 0 ? uses_nfw<KIND>(0, fs)
-                                    : slot == 1 ? uses_nfw<KIND>(1, fs) : uses_nfw<KIND>(2, fs);
+        const bool need = slot == 0 ? uses_nfw<kind_of_kv(KIND)>(0, fs)
+                                    : slot == 1 ? uses_nfw<kind_of_kv(KIND)>(1, fs) : uses_nfw<kind_of_kv(KIND)>(2, fs);
         if (need) {
           const Nfw nf = make_nfw(T, M, zt, f_of(a, fs));
           en.c[fs] = nf.c;
@@ -413,10 +426,10 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<KIND>
 // Pass 2: the inputs of u-factor item i (factor I or later).
 template <int KIND, class P, int I>
 __device__ __forceinline__ bool u_inputs(const DevTables &T,
-                                         const Stage<Spec<KIND>::D, KGroup<KIND>::G> &St,
+                                         const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
                                          const Cache<KIND> &C, int i, int ng, int &dst, Nfw &n,
                                          double &k) {
-  using S = Spec<KIND>;
+  using S = Spec<kind_of_kv(KIND)>;
   if constexpr (I >= S::NU) {
     return false;
   } else {
@@ -460,9 +473,9 @@ __device__ __forceinline__ bool u_inputs(const DevTables &T,
 }
 
 template <int KIND, class P>
-__device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<KIND>::D, KGroup<KIND>::G> &St,
+__device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
                                       Cache<KIND> &C, int i, int ng) {
-  using S = Spec<KIND>;
+  using S = Spec<kind_of_kv(KIND)>;
   constexpr int D = S::D, G = KGroup<KIND>::G;
   constexpr int NU_TOTAL = Cache<KIND>::NU_TOTAL;
   constexpr int UPAD = (NU_TOTAL + 63) / 64 * 64;
@@ -490,15 +503,15 @@ __device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<KIND>
 
 template <int KIND, class P>
 __global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
-  using S = Spec<KIND>;
+  using S = Spec<kind_of_kv(KIND)>;
   constexpr int D = S::D;
   constexpr int NP = KGroup<KIND>::P;
   constexpr int G = KGroup<KIND>::G;
   constexpr int NT = KGroup<KIND>::NT;
-  static_assert(P1Layout<KIND>::END <= NT, "pass 1 fits one item per thread");
+  static_assert(P1Layout<KIND>::END <= 2 * NT, "pass 1 fits two items per thread");
   constexpr int P2 = (Cache<KIND>::NU_TOTAL + 63) / 64 * 64 +
                      (S::has_bilin ? G * Cache<KIND>::NB : 0);
-  static_assert(P2 <= 2 * NT, "pass 2 fits two items per thread");
+  static_assert(P2 <= 3 * NT, "pass 2 fits three items per thread");
   static_assert(G * NP <= NT, "pass 3 fits one point per thread");
   __shared__ Stage<D, G> St;
   __shared__ Cache<KIND> C;
@@ -508,15 +521,17 @@ __global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
   const int tid = threadIdx.x;
   const DevTables T = L.T;
   pass1<KIND, P>(T, St, C, tid, ng);
+  if constexpr (P1Layout<KIND>::END > NT) pass1<KIND, P>(T, St, C, tid + NT, ng);
   __syncthreads();
   pass2<KIND, P>(T, St, C, tid, ng);
   if constexpr (P2 > NT) pass2<KIND, P>(T, St, C, tid + NT, ng);
+  if constexpr (P2 > 2 * NT) pass2<KIND, P>(T, St, C, tid + 2 * NT, ng);
   __syncthreads();
   if (tid < ng * NP) {
     const int g = tid / NP, q = tid - g * NP;
     if (St.slot[g] != 0xffffffffu) {
       CachedSrc<KIND, P> src(C, St.args[g], g, q);
-      s_f[tid] = integrand_src<KIND, P>(T, St.args[g], src);
+      s_f[tid] = integrand_src<kind_of_kv(KIND), P>(T, St.args[g], src);
     }
   }
   __syncthreads();
@@ -898,7 +913,7 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
 template <int KIND, class P>
 __global__ void k_points(DevTables T, const TermArgs *args, const double *x, uint32_t npts,
                          double *f) {
-  constexpr int D = kind_dim(KIND);
+  constexpr int D = kind_dim(kind_of_kv(KIND));
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npts) return;
   double p[D];
@@ -932,50 +947,58 @@ hipError_t launch_points(int kind, int prec, const DevTables &T, const TermArgs 
 }
 
 // ----------------------------------------------------------------- launch
-template <int KIND, class P>
+template <int KV, class P>
 static hipError_t launch_eval_kind(const EvalLaunch &L, hipStream_t s) {
-  using Gr = KGroup<KIND>;
+  using Gr = KGroup<KV>;
   uint32_t blocks = (L.count + Gr::G - 1) / Gr::G;
   if (blocks == 0) return hipSuccess;
-  if constexpr (KIND == kNbar) {
-    hipLaunchKernelGGL((k_eval_direct<KIND, P>), dim3(blocks), dim3(Gr::NT), 0, s, L);
+  if constexpr (kind_of_kv(KV) == kNbar) {
+    hipLaunchKernelGGL((k_eval_direct<KV, P>), dim3(blocks), dim3(Gr::NT), 0, s, L);
   } else {
-    hipLaunchKernelGGL((k_eval<KIND, P>), dim3(blocks), dim3(Gr::NT), 0, s, L);
+    hipLaunchKernelGGL((k_eval<KV, P>), dim3(blocks), dim3(Gr::NT), 0, s, L);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_eval(int kind, int prec, const EvalLaunch &L, hipStream_t s) {
-  if (prec == 0 || kind == kNbar) {
-    switch (kind) {
-      case kNbar: return launch_eval_kind<kNbar, P64>(L, s);
-      case kNN1: return launch_eval_kind<kNN1, P64>(L, s);
-      case kNN2: return launch_eval_kind<kNN2, P64>(L, s);
-      case kNN3: return launch_eval_kind<kNN3, P64>(L, s);
-      case kNM1: return launch_eval_kind<kNM1, P64>(L, s);
-      case kNM2: return launch_eval_kind<kNM2, P64>(L, s);
-      case kNM3: return launch_eval_kind<kNM3, P64>(L, s);
-      default: return hipErrorInvalidValue;
-    }
-  }
+template <int V, class P>
+static hipError_t launch_eval_variant(int kind, const EvalLaunch &L, hipStream_t s) {
   switch (kind) {
-    case kNN1: return launch_eval_kind<kNN1, PMix>(L, s);
-    case kNN2: return launch_eval_kind<kNN2, PMix>(L, s);
-    case kNN3: return launch_eval_kind<kNN3, PMix>(L, s);
-    case kNM1: return launch_eval_kind<kNM1, PMix>(L, s);
-    case kNM2: return launch_eval_kind<kNM2, PMix>(L, s);
-    case kNM3: return launch_eval_kind<kNM3, PMix>(L, s);
+    case kNN1: return launch_eval_kind<make_kv(kNN1, V), P>(L, s);
+    case kNN2: return launch_eval_kind<make_kv(kNN2, V), P>(L, s);
+    case kNN3: return launch_eval_kind<make_kv(kNN3, V), P>(L, s);
+    case kNM1: return launch_eval_kind<make_kv(kNM1, V), P>(L, s);
+    case kNM2: return launch_eval_kind<make_kv(kNM2, V), P>(L, s);
+    case kNM3: return launch_eval_kind<make_kv(kNM3, V), P>(L, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-int eval_group_size(int kind) {
-  switch (kind_dim(kind)) {
-    case 1: return KGroup<kNbar>::G;
-    case 5: return KGroup<kNN1>::G;
-    case 6: return KGroup<kNN2>::G;
-    default: return KGroup<kNN3>::G;
+hipError_t launch_eval(int kind, int prec, int variant, const EvalLaunch &L, hipStream_t s) {
+  if (kind == kNbar) return launch_eval_kind<kNbar, P64>(L, s);
+  if (variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
+  if (prec == 0) {
+    if (variant == 2) return launch_eval_variant<2, P64>(kind, L, s);
+    return variant ? launch_eval_variant<1, P64>(kind, L, s) : launch_eval_variant<0, P64>(kind, L, s);
   }
+  if (variant == 2) return launch_eval_variant<2, PMix>(kind, L, s);
+  return variant ? launch_eval_variant<1, PMix>(kind, L, s) : launch_eval_variant<0, PMix>(kind, L, s);
+}
+
+int eval_num_variants() { return kNumVariants; }
+
+int eval_group_size(int kind, int variant) {
+  const int d = kind_dim(kind);
+  if (d == 1) return KGroup<kNbar>::G;
+  const int k = d == 5 ? kNN1 : d == 6 ? kNN2 : kNN3;
+  if (variant == 2) {
+    return k == kNN1 ? KGroup<make_kv(kNN1, 2)>::G
+                     : k == kNN2 ? KGroup<make_kv(kNN2, 2)>::G : KGroup<make_kv(kNN3, 2)>::G;
+  }
+  if (variant == 1) {
+    return k == kNN1 ? KGroup<make_kv(kNN1, 1)>::G
+                     : k == kNN2 ? KGroup<make_kv(kNN2, 1)>::G : KGroup<make_kv(kNN3, 1)>::G;
+  }
+  return k == kNN1 ? KGroup<kNN1>::G : k == kNN2 ? KGroup<kNN2>::G : KGroup<kNN3>::G;
 }
 
 hipError_t launch_roots(const RootsLaunch &L, hipStream_t s) {

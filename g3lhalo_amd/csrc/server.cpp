@@ -298,7 +298,7 @@ int g3l_server_step(g3l_server *s, g3l_server_status *status) {
     if (!s->fill[k]) continue;
     EL.list = s->lists_k[k];
     EL.count = s->fill[k];
-    G3L_HIP(launch_eval(k, s->precision, EL, st));
+    G3L_HIP(launch_eval(k, s->precision, c->variant[s->precision == G3L_MIXED][k], EL, st));
   }
   G3L_HIP(launch_zero(dcnt, kNumKinds + 2, st));
   SelectLaunch SL{};

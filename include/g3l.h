@@ -148,6 +148,22 @@ typedef struct g3l_batch_stats {
 } g3l_batch_stats;
 int g3l_last_batch_stats(g3l_ctx *ctx, g3l_batch_stats *out);
 
+/* Block-shape tuning.  No counterpart in the reference (its CUDA build fixes
+ * <<<184,256>>>, src/NNMap_Model.cu:596-631).  The evaluation kernels come
+ * in launch variants (regions per block) that give identical results;
+ * g3l_autotune times each on a calibration batch (55 aperture triplets x
+ * 1h/2h/3h for both statistics, tol 1e-4, maxEval 5e6, HOD set `hodset`)
+ * and keeps the fastest per integrand kind for `precision`.  ms[k][v] is
+ * kind k's kernel time under variant v, group[k][v] its regions per block. */
+typedef struct g3l_tune_report {
+  int32_t variant[8];    /* chosen variant per kind (1-3 NNMap 1h-3h, 4-6 NMapMap) */
+  double ms[8][4];
+  int32_t group[8][4];
+} g3l_tune_report;
+int g3l_autotune(g3l_ctx *ctx, int32_t hodset, int32_t precision, g3l_tune_report *out);
+/* Sets one kind's variant directly (kind 1-6). */
+int g3l_set_variant(g3l_ctx *ctx, int32_t kind, int32_t precision, int32_t variant);
+
 /* Version string of this build. */
 const char *g3l_version(void);
 
