@@ -219,6 +219,10 @@ struct Cache {
   MassC m[S::NM][G][17];
   NfwC nf[cache_nfw<KIND>() ? S::NM : 1][cache_nfw<KIND>() ? G : 1][17];
   double pk[S::has_pk ? 2 : 1][G][17];  // P(k1), P(k2) per (l, z) class
+  static constexpr int N3 = ProjTab<D, kL3>::n;             // (l1, l2, phi) classes
+  static constexpr int N3Z = ProjTab<D, kL3 | bit(D - 1)>::n;  // (l1, l2, phi, z) classes
+  double l3[G][N3], ap3[G][N3];
+  double pk3[S::has_pk ? G : 1][N3Z];                      // P(k3), 2-halo terms
   double u[NU_TOTAL];
   double bl[G * NB];
 };
@@ -243,7 +247,7 @@ struct CachedSrc {
     G_.zr = C.zr[g][cz];
     G_.zt = C.zt[g][cz];
     G_.w = C.w[g][cz];
-    G_.l3 = sqrt(G_.l1 * G_.l1 + G_.l2 * G_.l2 + 2 * G_.l1 * G_.l2 * G_.cphi);
+    G_.l3 = C.l3[g][kProj<D, kL3>.cls[q]];
     G_.k1 = G_.l1 / G_.w;
     G_.k2 = G_.l2 / G_.w;
     G_.k3 = G_.l3 / G_.w;
@@ -290,6 +294,10 @@ struct CachedSrc {
   __device__ __forceinline__ double ap2() const { return C.ap2[g][kProj<D, kL2>.cls[q]]; }
   __device__ __forceinline__ double bl() const {
     return C.bl[g * Cache<KIND>::NB + kProj<D, bilin_mask<D>()>.cls[q]];
+  }
+  __device__ __forceinline__ double ap3() const { return C.ap3[g][kProj<D, kL3>.cls[q]]; }
+  __device__ __forceinline__ double pk3() const {
+    return C.pk3[S::has_pk ? g : 0][kProj<D, kL3 | bit(D - 1)>.cls[q]];
   }
   __device__ __forceinline__ double pk1() const {
     return C.pk[0][g][kProj<D, kL1 | bit(D - 1)>.cls[q]];
@@ -472,6 +480,20 @@ __device__ __forceinline__ bool u_inputs(const DevTables &T,
   }
 }
 
+// Pass 2 item layout: the u_NFW items, then (64-aligned) the bispectrum
+// items, l3 with its filter per (l1, l2, phi) class, and for the 2-halo terms
+// P(k3) per (l1, l2, phi, z) class.
+template <int KIND>
+struct P2Layout {
+  using S = Spec<kind_of_kv(KIND)>;
+  static constexpr int G = KGroup<KIND>::G;
+  static constexpr int pad(int n) { return (n + 63) / 64 * 64; }
+  static constexpr int UPAD = pad(Cache<KIND>::NU_TOTAL);
+  static constexpr int L3 = UPAD + (S::has_bilin ? pad(G * Cache<KIND>::NB) : 0);
+  static constexpr int PK3 = L3 + pad(G * Cache<KIND>::N3);
+  static constexpr int END = PK3 + (S::has_pk ? pad(G * Cache<KIND>::N3Z) : 0);
+};
+
 template <int KIND, class P>
 __device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
                                       Cache<KIND> &C, int i, int ng) {
@@ -487,17 +509,48 @@ __device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<kind_
     C.u[dst] = P::u(n, k);
     return;
   }
-  if constexpr (S::has_bilin) {
-    constexpr int NB = Cache<KIND>::NB;
-    const int r = i - UPAD;
-    if (r < 0 || r >= G * NB) return;
-    const int g = r / NB, c = r - g * NB;
+  using Lay = P2Layout<KIND>;
+  if (i < Lay::L3) {
+    if constexpr (S::has_bilin) {
+      constexpr int NB = Cache<KIND>::NB;
+      const int r = i - UPAD;
+      if (r < 0 || r >= G * NB) return;
+      const int g = r / NB, c = r - g * NB;
+      if (g >= ng || St.slot[g] == 0xffffffffu) return;
+      const int q = kProj<D, bilin_mask<D>()>.rep[c];
+      const int cz = kProj<D, bit(D - 1)>.cls[q];
+      const double w = C.w[g][cz];
+      const double k1 = C.l1[g][kProj<D, kL1>.cls[q]] / w, k2 = C.l2[g][kProj<D, kL2>.cls[q]] / w;
+      C.bl[r] = bilin(T, k1, k2, C.cphi[g][kProj<D, bit(2)>.cls[q]], C.z[g][cz]);
+    }
+    return;
+  }
+  if (i < Lay::PK3) {
+    // l3 and its filter per (l1, l2, phi) class, the expressions of the Geo
+    constexpr int N3 = Cache<KIND>::N3;
+    const int r = i - Lay::L3;
+    if (r >= G * N3) return;
+    const int g = r / N3, c = r - g * N3;
     if (g >= ng || St.slot[g] == 0xffffffffu) return;
-    const int q = kProj<D, bilin_mask<D>()>.rep[c];
+    const int q = kProj<D, kL3>.rep[c];
+    const double l1 = C.l1[g][kProj<D, kL1>.cls[q]], l2 = C.l2[g][kProj<D, kL2>.cls[q]];
+    const double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * C.cphi[g][kProj<D, bit(2)>.cls[q]]);
+    C.l3[g][c] = l3;
+    C.ap3[g][c] = P::ap(St.args[g].t3 * l3);
+    return;
+  }
+  if constexpr (S::has_pk) {
+    // P(k3) per (l1, l2, phi, z) class
+    constexpr int N3Z = Cache<KIND>::N3Z;
+    const int r = i - Lay::PK3;
+    if (r >= G * N3Z) return;
+    const int g = r / N3Z, c = r - g * N3Z;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return;
+    const int q = kProj<D, kL3 | bit(D - 1)>.rep[c];
     const int cz = kProj<D, bit(D - 1)>.cls[q];
-    const double w = C.w[g][cz];
-    const double k1 = C.l1[g][kProj<D, kL1>.cls[q]] / w, k2 = C.l2[g][kProj<D, kL2>.cls[q]] / w;
-    C.bl[r] = bilin(T, k1, k2, C.cphi[g][kProj<D, bit(2)>.cls[q]], C.z[g][cz]);
+    const double l1 = C.l1[g][kProj<D, kL1>.cls[q]], l2 = C.l2[g][kProj<D, kL2>.cls[q]];
+    const double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * C.cphi[g][kProj<D, bit(2)>.cls[q]]);
+    C.pk3[g][c] = pk(T, C.zr[g][cz], l3 / C.w[g][cz]);
   }
 }
 
@@ -509,8 +562,7 @@ __global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
   constexpr int G = KGroup<KIND>::G;
   constexpr int NT = KGroup<KIND>::NT;
   static_assert(P1Layout<KIND>::END <= 2 * NT, "pass 1 fits two items per thread");
-  constexpr int P2 = (Cache<KIND>::NU_TOTAL + 63) / 64 * 64 +
-                     (S::has_bilin ? G * Cache<KIND>::NB : 0);
+  constexpr int P2 = P2Layout<KIND>::END;
   static_assert(P2 <= 3 * NT, "pass 2 fits three items per thread");
   static_assert(G * NP <= NT, "pass 3 fits one point per thread");
   __shared__ Stage<D, G> St;

@@ -638,6 +638,8 @@ struct DirectSrc {
   G3L_HD double ap1() const { return P::ap(a.t1 * G.l1); }
   G3L_HD double ap2() const { return P::ap(a.t2 * G.l2); }
   G3L_HD double bl() const { return bilin(T, G.k1, G.k2, G.cphi, G.z); }
+  G3L_HD double ap3() const { return P::ap(a.t3 * G.l3); }
+  G3L_HD double pk3() const { return pk(T, G.zr, G.k3); }
   G3L_HD double pk1() const { return pk(T, G.zr, G.k1); }
   G3L_HD double pk2() const { return pk(T, G.zr, G.k2); }
   G3L_HD double zpre() const {
@@ -670,7 +672,7 @@ G3L_HD double nnmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   double B = s.zpre() * T.hmf[zi * T.nb + M.ir] * m * u3;
   B *= o1.nc * o2.ns * u2 + o2.nc * o1.ns * u1 + nss * u1 * u2;
   double b = B * s.zfac();
-  return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
+  return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b;
 }
 
 // NNMap 2-halo (src/NNMap_Model.cu:787-856).
@@ -679,7 +681,7 @@ G3L_HD double nnmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
   const Geo &G = s.geo();
   const Mass M1 = s.mass(0), M2 = s.mass(1);
   const int zi = G.zr, nb = T.nb;
-  double P1 = s.pk1(), P2 = s.pk2(), P3 = pk(T, zi, G.k3);
+  double P1 = s.pk1(), P2 = s.pk2(), P3 = s.pk3();
   // G_gg(k1, k2, m2)
   Occ a1 = s.occ1(1), a2 = s.occ2(1);
   double nss = s.nss(1);
@@ -702,7 +704,7 @@ G3L_HD double nnmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
              (m1 * u_k3_m1 * Ggg * P3 + m2 * u_k3_m2 * G_k1_m1_h1 * G_k2_m2_h2 * P1 +
               m2 * u_k3_m2 * G_k2_m1_h2 * G_k1_m2_h1 * P2);
   double b = s.zfac() * B;
-  return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
+  return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b;
 }
 
 // NNMap 3-halo (src/NNMap_Model.cu:1003-1037).
@@ -721,7 +723,7 @@ G3L_HD double nnmap_3h(const DevTables &T, const TermArgs &a, const S &s) {
              T.hmf[zi * nb + M2.ir] * T.hmf[zi * nb + M3.ir] * m3 * u3 * G1 * G2 * s.bl() *
              T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] * T.bh[zi * nb + M3.ir];
   double b = B * s.zfac();
-  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b;
+  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b;
 }
 
 // NMapMap 1-halo (src/NMapMap_Model.cu:263-301).
@@ -738,7 +740,7 @@ G3L_HD double nmapmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   double B = s.zpre() * T.hmf[zi * T.nb + M.ir] * m * m * (o.nc + o.ns * uf) *
              s.template u<1>() * P::u(n1, G.k3);
   double b = B * s.zfac();
-  return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
+  return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b / nbar;
 }
 
 // NMapMap 2-halo (src/NMapMap_Model.cu:402-468).
@@ -748,7 +750,7 @@ G3L_HD double nmapmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
   const Mass M1 = s.mass(0), M2 = s.mass(1);
   const int zi = G.zr, nb = T.nb;
   double nbar = a.nb1[zi];
-  double P1 = s.pk1(), P2 = s.pk2(), P3 = pk(T, zi, G.k3);
+  double P1 = s.pk1(), P2 = s.pk2(), P3 = s.pk3();
   Occ o1 = s.occ1(0), o2 = s.occ1(1);
   double u_k2_m1 = s.template u<0>(), u_k3_m1 = s.template u<1>();
   double u_k2_m2 = s.template u<2>(), u_k3_m2 = s.template u<3>();
@@ -761,7 +763,7 @@ G3L_HD double nmapmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
               m1 * m2 * u_k3_m1 * u_k2_m2 * (o2.nc + o2.ns * u_k1_m2_f) * P3 +
               m2 * m2 * u_k2_m1 * u_k3_m2 * (o1.nc + o1.ns * u_k1_m1_f) * P1);
   double b = B * s.zfac();
-  return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
+  return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b / nbar;
 }
 
 // NMapMap 3-halo: Ncen, Nsat at m3 with u_NFW(k1, m2, f) (src/NMapMap_Model.cu:593-632).
@@ -782,7 +784,7 @@ G3L_HD double nmapmap_3h(const DevTables &T, const TermArgs &a, const S &s) {
              T.bh[zi * nb + M3.ir] * m1 * m2 * u_k2_m1 * u_k3_m2 * (o.nc + o.ns * u_k1_m2_f) *
              s.bl();
   double b = B * s.zfac();
-  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * P::ap(a.t3 * G.l3) * b / nbar;
+  return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b / nbar;
 }
 
 // The integrand of kind KIND at one point (direct evaluation).
