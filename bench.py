@@ -36,9 +36,19 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this workload scaled by a copy kernel of known size
-# (profiles/k_eval_nnmap550_counters.md): 569.4 MB read + 32.5 MB written per
+# (profiles/k_eval_nnmap550_counters.md): 464.1 MB read + 43.1 MB written per
 # k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
-TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 601915309}
+TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 507252858}
+
+
+# Launch variant per kind when the autotuner does not run (the library's
+# built-in choice); the kernel's template argument is kind + 16 x variant.
+DEFAULT_VARIANT = {0: 0, 1: 1, 2: 2, 3: 2, 4: 1, 5: 1, 6: 2}
+
+
+def kernel_kv(tuned: dict, prec: str, kind: int) -> int:
+    v = tuned[prec][kind]["variant"] if prec in tuned else DEFAULT_VARIANT[kind]
+    return kind + 16 * v
 
 
 def workload(name: str, hodset: int = 0):
@@ -184,6 +194,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests in the served run (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=4.0,
@@ -204,10 +215,13 @@ def main(argv=None):
     sid = ctx.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
     reqs, desc = workload(a.workload, sid)
     # block shapes per integrand kind, timed on this device (results do not
-    # depend on them)
-    tuned = {"fp64": ctx.autotune(sid, _lib.FP64)}
-    if not a.fp64_only:
-        tuned["mixed"] = ctx.autotune(sid, _lib.MIXED)
+    # depend on them); --no-autotune keeps the built-in choices (profiling
+    # runs, so that every launch of a kernel belongs to the timed steps)
+    tuned = {}
+    if not a.no_autotune:
+        tuned["fp64"] = ctx.autotune(sid, _lib.FP64)
+        if not a.fp64_only:
+            tuned["mixed"] = ctx.autotune(sid, _lib.MIXED)
 
     def run_form(precision):
         for _ in range(a.warmup):
@@ -278,7 +292,8 @@ def main(argv=None):
                 "p50": float(np.quantile(mdev, 0.5)), "p90": float(np.quantile(mdev, 0.9)),
                 "p99": float(np.quantile(mdev, 0.99)), "max": float(mdev.max()),
                 "frac_above_1e-4": float(np.mean(mdev > 1e-4))},
-            "roofline": {"kernel": f"k_eval<{mk},PMix> ({KIND_NAMES[mk]})",
+            "roofline": {"kernel": f"k_eval<{kernel_kv(tuned, 'mixed', mk)},PMix> "
+                                   f"({KIND_NAMES[mk]})",
                          "achieved": round(mach, 3), "peak": round(flops.mixed_peak_tflops(mk), 2),
                          "unit": "TFLOP/s",
                          "frac": round(mach / flops.mixed_peak_tflops(mk), 4)},
@@ -309,12 +324,14 @@ def main(argv=None):
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
                        "p99": round(float(np.percentile(step_ms, 99)), 3),
                        "note": "batched form: every integral of a step completes with the step"},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
+                     "compute": "FP64 vector ALU: no contraction maps onto MFMA; MI355X's "
+                                "FP64 vector and matrix peaks are both 78.6 TFLOP/s",
                      "peak": round(peak, 2), "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((a.workload, dom)),
-                     "kernel": f"k_eval<{dom},P64> ({KIND_NAMES[dom]}, fused points + "
-                               "integrand + rule)",
+                     "kernel": f"k_eval<{kernel_kv(tuned, 'fp64', dom)},P64> "
+                               f"({KIND_NAMES[dom]}, fused points + integrand + rule)",
                      "avg_launch_ms": round(float(kind_ms[dom] / max(kind_launches[dom], 1)), 4),
                      "per_kernel": per_kind},
         "autotune": {p: {KIND_NAMES[k]: r["group"][r["variant"]] for k, r in t.items()}
