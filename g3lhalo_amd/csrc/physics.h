@@ -52,6 +52,54 @@ G3L_HD double fdiv(double a, double b) {
 #endif
 }
 
+// sin and cos together for |x| < 2^20; larger arguments take the library's
+// sincos.  x = n pi/2 + r by Cody and Waite's additive reduction (Cody &
+// Waite, "Software Manual for the Elementary Functions", 1980) in three FMA
+// steps (pi/2 split into
+// three doubles, so n pi/2 is carried to ~1e-49 relative and r has an
+// absolute error near 1e-16), |r| <= pi/4 up to rounding; then the Taylor
+// series of sin r through r^17 and of cos r through r^16, whose truncation
+// errors at |r| = pi/4 are below 1e-19 and 3e-18, summed over precomputed
+// powers of r^2 (coefficients as FMA multiplicands, in scalar registers);
+// n mod 4 selects and signs the pair.  The NFW arguments k r_s stay far below
+// 2^20 in every workload here.
+G3L_HD void sincos_cw(double x, double &s, double &c) {
+  if (!(fabs(x) < 1048576.0)) {
+    sincos(x, &s, &c);
+    return;
+  }
+  const double n = rint(x * 0.6366197723675814);
+  double r = fma(-n, 1.5707963267948966, x);
+  r = fma(-n, 6.123233995736766e-17, r);
+  r = fma(-n, -1.4973849048591698e-33, r);
+  const double z = r * r;
+  const double z2 = z * z, z3 = z2 * z, z4 = z2 * z2;
+  const double z5 = z4 * z, z6 = z3 * z3, z7 = z4 * z3, z8 = z4 * z4;
+  double ps = -0.16666666666666666 * z;
+  ps = fma(0.008333333333333333, z2, ps);
+  ps = fma(-0.0001984126984126984, z3, ps);
+  ps = fma(2.7557319223985893e-06, z4, ps);
+  ps = fma(-2.505210838544172e-08, z5, ps);
+  ps = fma(1.6059043836821613e-10, z6, ps);
+  ps = fma(-7.647163731819816e-13, z7, ps);
+  ps = fma(2.8114572543455206e-15, z8, ps);
+  double pc = -0.5 * z;
+  pc = fma(0.041666666666666664, z2, pc);
+  pc = fma(-0.001388888888888889, z3, pc);
+  pc = fma(2.48015873015873e-05, z4, pc);
+  pc = fma(-2.755731922398589e-07, z5, pc);
+  pc = fma(2.08767569878681e-09, z6, pc);
+  pc = fma(-1.1470745597729725e-11, z7, pc);
+  pc = fma(4.779477332387385e-14, z8, pc);
+  const double sr = fma(r, ps, r);
+  const double cr = 1.0 + pc;
+  const int q = (int)n;
+  const double a = (q & 1) ? cr : sr;
+  const double b = (q & 1) ? sr : cr;
+  s = (q & 2) ? -a : a;
+  c = ((q + 1) & 2) ? -b : b;
+}
+
 // Tables resident in HBM (z-major 2-D tables, nb*nb + nb for P_lin).
 struct DevTables {
   int nb;
@@ -138,63 +186,103 @@ struct Occ {
 
 // ----------------------------------------------------------------- Si, Ci
 // Rational approximations of src/generalFunctions.cu:44-87.
+//
+// The polynomials are summed term by term over precomputed powers,
+// acc = fma(c_k, t^k, acc), rather than by Horner's rule.  In Horner's form
+// every coefficient is an addend, and this target's two-address FP64 FMA
+// needs an addend in vector registers (two v_mov per coefficient); as a
+// multiplicand a coefficient stays in scalar registers, so each term costs
+// one vector instruction.  The powers are built by a balanced product tree
+// (at most four roundings for t^10); every sum below has terms of one sign
+// or a negligible opposite-sign tail, so the result stays within a few
+// units in the last place of the reference's left-to-right form.
+
 // Si(x) and the rational part of Ci(x) = gamma + ln x + cr for x < 4.
 G3L_HD void sici_small_parts(double x, double &si, double &cr) {
-  double x2 = x * x;
-  double x4 = x2 * x2;
-  double x6 = x2 * x4;
-  double x8 = x4 * x4;
-  double x10 = x8 * x2;
-  double x12 = x6 * x6;
-  double x14 = x12 * x2;
-  double a = 1 - 4.54393409816329991e-2 * x2 + 1.15457225751016682e-3 * x4 -
-             1.41018536821330254e-5 * x6 + 9.43280809438713025e-8 * x8 -
-             3.53201978997168357e-10 * x10 + 7.08240282274875911e-13 * x12 -
-             6.05338212010422477e-16 * x14;
-  double b = 1 + 1.01162145739225565e-2 * x2 + 4.99175116169755106e-5 * x4 +
-             1.55654986308745614e-7 * x6 + 3.28067571055789734e-10 * x8 +
-             4.5049097575386581e-13 * x10 + 3.21107051193712168e-16 * x12;
+  const double z = x * x;
+  const double z2 = z * z, z3 = z2 * z, z4 = z2 * z2;
+  const double z5 = z4 * z, z6 = z3 * z3, z7 = z4 * z3;
+  double a = fma(-4.54393409816329991e-2, z, 1.0);
+  a = fma(1.15457225751016682e-3, z2, a);
+  a = fma(-1.41018536821330254e-5, z3, a);
+  a = fma(9.43280809438713025e-8, z4, a);
+  a = fma(-3.53201978997168357e-10, z5, a);
+  a = fma(7.08240282274875911e-13, z6, a);
+  a = fma(-6.05338212010422477e-16, z7, a);
+  double b = fma(1.01162145739225565e-2, z, 1.0);
+  b = fma(4.99175116169755106e-5, z2, b);
+  b = fma(1.55654986308745614e-7, z3, b);
+  b = fma(3.28067571055789734e-10, z4, b);
+  b = fma(4.5049097575386581e-13, z5, b);
+  b = fma(3.21107051193712168e-16, z6, b);
   si = fdiv(x * a, b);
-  a = -0.25 + 7.51851524438898291e-3 * x2 - 1.27528342240267686e-4 * x4 +
-      1.05297363846239184e-6 * x6 - 4.68889508144848019e-9 * x8 +
-      1.06480802891189243e-11 * x10 - 9.93728488857585407e-15 * x12;
-  b = 1 + 1.1592605689110735e-2 * x2 + 6.72126800814254432e-5 * x4 +
-      2.55533277086129636e-7 * x6 + 6.97071295760958946e-10 * x8 +
-      1.38536352772778619e-12 * x10 + 1.89106054713059759e-15 * x12 +
-      1.39759616731376855e-18 * x14;
-  cr = fdiv(x2 * a, b);
+  // x^2 times Ci's numerator, -x^2/4 + ..., summed from its first term
+  double n = -0.25 * z;
+  n = fma(7.51851524438898291e-3, z2, n);
+  n = fma(-1.27528342240267686e-4, z3, n);
+  n = fma(1.05297363846239184e-6, z4, n);
+  n = fma(-4.68889508144848019e-9, z5, n);
+  n = fma(1.06480802891189243e-11, z6, n);
+  n = fma(-9.93728488857585407e-15, z7, n);
+  double d = fma(1.1592605689110735e-2, z, 1.0);
+  d = fma(6.72126800814254432e-5, z2, d);
+  d = fma(2.55533277086129636e-7, z3, d);
+  d = fma(6.97071295760958946e-10, z4, d);
+  d = fma(1.38536352772778619e-12, z5, d);
+  d = fma(1.89106054713059759e-15, z6, d);
+  d = fma(1.39759616731376855e-18, z7, d);
+  cr = fdiv(n, d);
 }
 
 // g(x) and f(x) - 1/x for x >= 4, the latter from the exact coefficient
-// differences of f's numerator and denominator (no cancellation).
+// differences of f's numerator and denominator (no cancellation); t = 1/x^2.
 G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
-  double t = fdiv(1.0, x * x);
-  double fd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
-                  1.11535493509914254097e13, 1.43468549171581016479e13),
-                  5.06084464593475076774e12), 7.08501308149515401563e11),
-                  4.58595115847765779830e10), 1.47478952192985464958e9),
-                  2.41535670165126845144e7), 1.97865247031583951450e5),
-                  7.46437068161927678031e2);
-  double dn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
-                  -4.94701168645415954590e11, -6.20538246899190625000e12),
-                  -4.26733661912124414062e12), -8.51162840223981323242e11),
-                  -6.79674775754933776855e10), -2.48588769773332548141e9),
-                  -4.40554837171082794666e7), -3.78536003969501121901e5),
-                  -1.46887413643708168820e3), -1.99999999999097743952e0);
-  double gn = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
-                  -1.36517137670871689e12, 6.43291613143049485e12),
-                  1.81004487464664575e13), 7.57664583257834349e12),
-                  1.09049528450362786e12), 6.83052205423625007e10),
-                  2.06297595146763354e9), 3.12557570795778731e7),
-                  2.35239181626478200e5), 8.1359520115168615e2);
-  double gd = fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t, fma(t,
-                  3.99653257887490811e13, 4.01839087307656620e13),
-                  1.17164723371736605e13), 1.39866710696414565e12),
-                  7.87465017341829930e10), 2.23355543278099360e9),
-                  3.26026661647090822e7), 2.40036752835578777e5),
-                  8.19595201151451564e2);
-  g = fdiv((1 + t * gn) * t, 1 + t * gd);
-  fm = fdiv(t * dn, (1 + t * fd) * x);
+  const double t = fdiv(1.0, x * x);
+  const double t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t4 * t;
+  const double t6 = t3 * t3, t7 = t4 * t3, t8 = t4 * t4, t9 = t5 * t4, t10 = t5 * t5;
+  // 1 + t fd(t): f's denominator
+  double fd = fma(7.46437068161927678031e2, t, 1.0);
+  fd = fma(1.97865247031583951450e5, t2, fd);
+  fd = fma(2.41535670165126845144e7, t3, fd);
+  fd = fma(1.47478952192985464958e9, t4, fd);
+  fd = fma(4.58595115847765779830e10, t5, fd);
+  fd = fma(7.08501308149515401563e11, t6, fd);
+  fd = fma(5.06084464593475076774e12, t7, fd);
+  fd = fma(1.43468549171581016479e13, t8, fd);
+  fd = fma(1.11535493509914254097e13, t9, fd);
+  // t dn(t): numerator minus denominator of f, times t
+  double dn = -1.99999999999097743952e0 * t;
+  dn = fma(-1.46887413643708168820e3, t2, dn);
+  dn = fma(-3.78536003969501121901e5, t3, dn);
+  dn = fma(-4.40554837171082794666e7, t4, dn);
+  dn = fma(-2.48588769773332548141e9, t5, dn);
+  dn = fma(-6.79674775754933776855e10, t6, dn);
+  dn = fma(-8.51162840223981323242e11, t7, dn);
+  dn = fma(-4.26733661912124414062e12, t8, dn);
+  dn = fma(-6.20538246899190625000e12, t9, dn);
+  dn = fma(-4.94701168645415954590e11, t10, dn);
+  // 1 + t gn(t) and 1 + t gd(t): g's numerator and denominator
+  double gn = fma(8.1359520115168615e2, t, 1.0);
+  gn = fma(2.35239181626478200e5, t2, gn);
+  gn = fma(3.12557570795778731e7, t3, gn);
+  gn = fma(2.06297595146763354e9, t4, gn);
+  gn = fma(6.83052205423625007e10, t5, gn);
+  gn = fma(1.09049528450362786e12, t6, gn);
+  gn = fma(7.57664583257834349e12, t7, gn);
+  gn = fma(1.81004487464664575e13, t8, gn);
+  gn = fma(6.43291613143049485e12, t9, gn);
+  gn = fma(-1.36517137670871689e12, t10, gn);
+  double gd = fma(8.19595201151451564e2, t, 1.0);
+  gd = fma(2.40036752835578777e5, t2, gd);
+  gd = fma(3.26026661647090822e7, t3, gd);
+  gd = fma(2.23355543278099360e9, t4, gd);
+  gd = fma(7.87465017341829930e10, t5, gd);
+  gd = fma(1.39866710696414565e12, t6, gd);
+  gd = fma(1.17164723371736605e13, t7, gd);
+  gd = fma(4.01839087307656620e13, t8, gd);
+  gd = fma(3.99653257887490811e13, t9, gd);
+  g = fdiv(gn * t, gd);
+  fm = fdiv(dn, fd * x);
 }
 
 // ======================================================================
@@ -237,10 +325,10 @@ struct P64 {
     const double a1 = fdiv(k * n.r200, n.c);
     const double a2 = a1 * (1 + n.c);
     double sx, cx;
-    sincos(a1 * n.c, &sx, &cx);
+    sincos_cw(a1 * n.c, sx, cx);
     double s1 = 0, c1 = 0, si1 = 0, cr1 = 0, g2 = 0, fm2 = 0;
     if (a1 < 4) {
-      sincos(a1, &s1, &c1);
+      sincos_cw(a1, s1, c1);
       sici_small_parts(a1, si1, cr1);
     }
     if (a2 >= 4) sici_aux_gfm(a2, g2, fm2);

@@ -43,6 +43,38 @@ const double kC = 2.99792e5;
 const double kGMsun = 1.327e11;
 const double kMpcKm = 3.086e19;
 
+/* Sensitivity probe, for tests only.  With a nonzero seed every result of
+ * exp, log, pow, sin, cos and erf inside the integrands moves by one unit in
+ * the last place -- down for a quarter of the arguments, up for another
+ * quarter, chosen by a hash of the argument bits and the seed -- which stands
+ * in for a different, equally accurate math library.  An integral that stops
+ * at the evaluation cap unconverged can refine differently under such a
+ * change; the spread it causes is the scale against which a second
+ * implementation's values are judged.  Seed 0 (the default) changes nothing. */
+uint64_t g_ulp_seed = 0;
+
+inline double ulp_jitter(double r, double a, double b) {
+  if (g_ulp_seed == 0 || !std::isfinite(r) || r == 0) return r;
+  uint64_t ua, ub;
+  std::memcpy(&ua, &a, 8);
+  std::memcpy(&ub, &b, 8);
+  uint64_t h = (ua ^ (ub * 0xD6E8FEB86659FD93ull) ^ g_ulp_seed) * 0x9E3779B97F4A7C15ull;
+  h ^= h >> 31;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 29;
+  switch (h >> 62) {
+    case 0: return std::nextafter(r, -INFINITY);
+    case 1: return std::nextafter(r, INFINITY);
+    default: return r;
+  }
+}
+inline double Jexp(double x) { return ulp_jitter(std::exp(x), x, 1); }
+inline double Jlog(double x) { return ulp_jitter(std::log(x), x, 2); }
+inline double Jpow(double x, double y) { return ulp_jitter(std::pow(x, y), x, y); }
+inline double Jsin(double x) { return ulp_jitter(std::sin(x), x, 3); }
+inline double Jcos(double x) { return ulp_jitter(std::cos(x), x, 4); }
+inline double Jerf(double x) { return ulp_jitter(std::erf(x), x, 5); }
+
 /* ---------------------------------------------------------------------- */
 /* Adaptive cubature (restatement of src/hcubature.cpp:115-1067)          */
 /* ---------------------------------------------------------------------- */
@@ -425,13 +457,13 @@ void sici(double x, double &si, double &ci) {
         2.55533277086129636e-7 * x6 + 6.97071295760958946e-10 * x8 +
         1.38536352772778619e-12 * x10 + 1.89106054713059759e-15 * x12 +
         1.39759616731376855e-18 * x14;
-    ci = gamma + std::log(x) + x2 * a / b;
+    ci = gamma + Jlog(x) + x2 * a / b;
   } else {
     double x16 = x8 * x8;
     double x18 = x16 * x2;
     double x20 = x10 * x10;
-    double cx = std::cos(x);
-    double sx = std::sin(x);
+    double cx = Jcos(x);
+    double sx = Jsin(x);
     double f = (1 + 7.44437068161936700618e2 / x2 + 1.96396372895146869801e5 / x4 +
                 2.37750310125431834034e7 / x6 + 1.43073403821274636888e9 / x8 +
                 4.33736238870432522765e10 / x10 + 6.40533830574022022911e11 / x12 +
@@ -460,22 +492,22 @@ void sici(double x, double &si, double &ci) {
 }
 
 /* U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42). */
-inline double aperture(double eta) { return 0.5 * eta * eta * std::exp(-0.5 * eta * eta); }
+inline double aperture(double eta) { return 0.5 * eta * eta * Jexp(-0.5 * eta * eta); }
 
 /* HOD occupations (src/HOD.cu:35-94). */
 inline double ncen(const Hod &h, double m) {
-  return 0.5 * h.alpha * (1 + std::erf(std::log(m / h.mth) / h.sigma / 1.414213562));
+  return 0.5 * h.alpha * (1 + Jerf(Jlog(m / h.mth) / h.sigma / 1.414213562));
 }
 inline double nsat(const Hod &h, double m) {
-  return 0.5 * (1 + std::erf(std::log(m / h.mth) / h.sigma / 1.414213562)) *
-         std::pow(m / h.mprime, h.beta);
+  return 0.5 * (1 + Jerf(Jlog(m / h.mth) / h.sigma / 1.414213562)) *
+         Jpow(m / h.mprime, h.beta);
 }
 inline double nsatnsat(const Hod &h1, const Hod &h2, bool same, double A, double eps, double m) {
   const double scale1 = 1;
   double ns1 = nsat(h1, m);
   if (same) return (ns1 + scale1 * scale1 - 1) * ns1;
   double ns2 = nsat(h2, m);
-  double r = ns1 * ns2 + A * std::pow(m, eps) * std::sqrt(ns1 * ns2);
+  double r = ns1 * ns2 + A * Jpow(m, eps) * std::sqrt(ns1 * ns2);
   if (r < 0) return 0;
   return r;
 }
@@ -503,10 +535,10 @@ struct Model {
    * int() truncation inside u_NFW and r_200 (src/generalFunctions.cu:11-12,35). */
   inline int zix_round(double z) const { return (int)std::round((z - zmin) * nb / (zmax - zmin)); }
   inline int mix_round(double m) const {
-    return (int)std::round(std::log(m / mmin) * nb / std::log(mmax / mmin));
+    return (int)std::round(Jlog(m / mmin) * nb / Jlog(mmax / mmin));
   }
   inline int kix_round(double k) const {
-    return (int)std::round(std::log(k / kmin) * nb / std::log(kmax / kmin));
+    return (int)std::round(Jlog(k / kmin) * nb / Jlog(kmax / kmin));
   }
 
   /* P(k) look-up (src/NNMap_Model.cu:819-839): below kmin linear in k; no
@@ -519,23 +551,23 @@ struct Model {
 
   double r200(double m, double z) const {
     int zi = int((z - zmin) * nb / (zmax - zmin));
-    return std::pow(0.239 * m / rho_bar[zi] / 200, 1. / 3.);
+    return Jpow(0.239 * m / rho_bar[zi] / 200, 1. / 3.);
   }
 
   /* Truncated-NFW Fourier profile (src/generalFunctions.cu:6-30). */
   double unfw(double k, double m, double z, double f) const {
     int zi = int((z - zmin) * nb / (zmax - zmin));
-    int mi = int(std::log(m / mmin) * nb / std::log(mmax / mmin));
+    int mi = int(Jlog(m / mmin) * nb / Jlog(mmax / mmin));
     double c = f * conc[(size_t)zi * nb + mi];
     double arg1 = k * r200(m, z) / c;
     double arg2 = arg1 * (1 + c);
     double si1, ci1, si2, ci2;
     sici(arg1, si1, ci1);
     sici(arg2, si2, ci2);
-    double term1 = std::sin(arg1) * (si2 - si1);
-    double term2 = std::cos(arg1) * (ci2 - ci1);
-    double term3 = -std::sin(arg1 * c) / arg2;
-    double F = std::log(1. + c) - c / (1. + c);
+    double term1 = Jsin(arg1) * (si2 - si1);
+    double term2 = Jcos(arg1) * (ci2 - ci1);
+    double term3 = -Jsin(arg1 * c) / arg2;
+    double F = Jlog(1. + c) - c / (1. + c);
     return (term1 + term2 + term3) / F;
   }
 
@@ -596,7 +628,7 @@ double nnm_1h(const TermCtx &T, double l1, double l2, double phi, double m, doub
   int mi = M.mix_round(m);
   double nbar1 = T.nb1[zi], nbar2 = T.nb2[zi];
   double w_ = M.w[zi];
-  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * Jcos(phi));
   double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
   double B = 1. / nbar1 / nbar2 / M.rho_bar[zi] * M.hmf[(size_t)zi * M.nb + mi] * m *
              M.unfw(k3, m, z, 1);
@@ -615,7 +647,7 @@ double nnm_2h(const TermCtx &T, double l1, double l2, double phi, double m1, dou
   int i1 = M.mix_round(m1), i2 = M.mix_round(m2);
   double nbar1 = T.nb1[zi], nbar2 = T.nb2[zi];
   double w_ = M.w[zi];
-  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * Jcos(phi));
   double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
   double P1 = M.pk(zi, M.kix_round(k1), k1);
   double P2 = M.pk(zi, M.kix_round(k2), k2);
@@ -640,11 +672,11 @@ double nnm_3h(const TermCtx &T, double l1, double l2, double phi, double m1, dou
   int i1 = M.mix_round(m1), i2 = M.mix_round(m2), i3 = M.mix_round(m3);
   double nbar1 = T.nb1[zi], nbar2 = T.nb2[zi];
   double w_ = M.w[zi];
-  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * Jcos(phi));
   double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
   double B = 1. / nbar1 / nbar2 / M.rho_bar[zi] * M.hmf[zi * nb + i1] * M.hmf[zi * nb + i2] *
              M.hmf[zi * nb + i3] * m3 * M.unfw(k3, m3, z, 1) * M.Gg(k1, m1, z, *T.h1) *
-             M.Gg(k2, m2, z, *T.h2) * M.bilin(k1, k2, std::cos(phi), z) * M.bh[zi * nb + i1] *
+             M.Gg(k2, m2, z, *T.h2) * M.bilin(k1, k2, Jcos(phi), z) * M.bh[zi * nb + i1] *
              M.bh[zi * nb + i2] * M.bh[zi * nb + i3];
   double b = B / M.dwdz[zi] * M.g[zi] * M.p1[zi] * M.p2[zi] / w_ / w_ / w_ * (1. + z);
   return l1 * l2 * m1 * m2 * m3 * l1 * l2 * aperture(T.t1 * l1) * aperture(T.t2 * l2) *
@@ -658,7 +690,7 @@ double nmm_1h(const TermCtx &T, double l1, double l2, double phi, double m, doub
   int mi = M.mix_round(m);
   double nbar = T.nb1[zi];
   double w_ = M.w[zi];
-  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * Jcos(phi));
   double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
   const Hod &h = *T.h1;
   double Nc = ncen(h, m), Ns = nsat(h, m);
@@ -679,7 +711,7 @@ double nmm_2h(const TermCtx &T, double l1, double l2, double phi, double m1, dou
   int i1 = M.mix_round(m1), i2 = M.mix_round(m2);
   double nbar = T.nb1[zi];
   double w_ = M.w[zi];
-  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * Jcos(phi));
   double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
   double P1 = M.pk(zi, M.kix_round(k1), k1);
   double P2 = M.pk(zi, M.kix_round(k2), k2);
@@ -711,7 +743,7 @@ double nmm_3h(const TermCtx &T, double l1, double l2, double phi, double m1, dou
   int i1 = M.mix_round(m1), i2 = M.mix_round(m2), i3 = M.mix_round(m3);
   double nbar = T.nb1[zi];
   double w_ = M.w[zi];
-  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+  double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * Jcos(phi));
   double k1 = l1 / w_, k2 = l2 / w_, k3 = l3 / w_;
   const Hod &h = *T.h1;
   double Nc = ncen(h, m3), Ns = nsat(h, m3);
@@ -719,7 +751,7 @@ double nmm_3h(const TermCtx &T, double l1, double l2, double phi, double m1, dou
   double B = 1. / nbar / rho / rho * M.hmf[zi * nb + i1] * M.hmf[zi * nb + i2] *
              M.hmf[zi * nb + i3] * M.bh[zi * nb + i1] * M.bh[zi * nb + i2] *
              M.bh[zi * nb + i3] * m1 * m2 * M.unfw(k2, m1, z, 1) * M.unfw(k3, m2, z, 1) *
-             (Nc + Ns * M.unfw(k1, m2, z, h.f)) * M.bilin(k1, k2, std::cos(phi), z);
+             (Nc + Ns * M.unfw(k1, m2, z, h.f)) * M.bilin(k1, k2, Jcos(phi), z);
   double b = B / M.dwdz[zi] * M.g[zi] * M.g[zi] * M.p1[zi] / w_ / w_ * (1. + z) * (1. + z);
   return l1 * l2 * m1 * m2 * m3 * l1 * l2 * aperture(T.t1 * l1) * aperture(T.t2 * l2) *
          aperture(T.t3 * l3) * b / nbar;
@@ -739,31 +771,31 @@ int term_cb(unsigned ndim, size_t npts, const double *x, void *fd, unsigned fdim
     double r = 0;
     switch (key) {
       case 0: { /* n_bar */
-        double m = std::pow(10, p[0]);
+        double m = Jpow(10, p[0]);
         r = T.M->k_nbar(m, T.z, *T.h1);
         break;
       }
       case 1:
-        r = nnm_1h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]), p[4]);
+        r = nnm_1h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]), p[4]);
         break;
       case 2:
-        r = nnm_2h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]),
-                   std::pow(10, p[4]), p[5]);
+        r = nnm_2h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]),
+                   Jpow(10, p[4]), p[5]);
         break;
       case 3:
-        r = nnm_3h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]),
-                   std::pow(10, p[4]), std::pow(10, p[5]), p[6]);
+        r = nnm_3h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]),
+                   Jpow(10, p[4]), Jpow(10, p[5]), p[6]);
         break;
       case 5:
-        r = nmm_1h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]), p[4]);
+        r = nmm_1h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]), p[4]);
         break;
       case 6:
-        r = nmm_2h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]),
-                   std::pow(10, p[4]), p[5]);
+        r = nmm_2h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]),
+                   Jpow(10, p[4]), p[5]);
         break;
       case 7:
-        r = nmm_3h(T, std::pow(10, p[0]), std::pow(10, p[1]), p[2], std::pow(10, p[3]),
-                   std::pow(10, p[4]), std::pow(10, p[5]), p[6]);
+        r = nmm_3h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]),
+                   Jpow(10, p[4]), Jpow(10, p[5]), p[6]);
         break;
       default:
         r = 0;
@@ -983,6 +1015,8 @@ int orc_model_borders(void *model, double *out4) {
   out4[0] = M.zmin_i; out4[1] = M.zmax_i; out4[2] = M.mmin_i; out4[3] = M.mmax_i;
   return 0;
 }
+
+void orc_set_ulp_seed(uint64_t seed) { g_ulp_seed = seed; }
 
 int orc_eval_term(void *model, int stat, int term, double theta1, double theta2,
                   double theta3, int type1, int type2, double reltol, uint64_t maxeval,

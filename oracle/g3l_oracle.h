@@ -108,6 +108,11 @@ int orc_rule_one_region(unsigned dim, orc_integrand_v f, void *fdata, const doub
                         const double *halfwidth, double *val, double *err, unsigned *split);
 uint32_t orc_points_per_region(unsigned dim);
 
+/* Sensitivity probe (tests only): a nonzero seed moves every exp, log, pow,
+ * sin, cos and erf result inside the integrands by at most one unit in the
+ * last place, by a hash of the argument; 0 restores the exact functions. */
+void orc_set_ulp_seed(uint64_t seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,8 @@ def lib():
         L.orc_fkernel.argtypes = [C.c_double] * 3
         L.orc_points_per_region.restype = C.c_uint32
         L.orc_points_per_region.argtypes = [C.c_uint]
+        L.orc_set_ulp_seed.restype = None
+        L.orc_set_ulp_seed.argtypes = [C.c_uint64]
         _lib = L
     return _lib
 
@@ -89,6 +91,13 @@ def reference_hcubature():
             return None
         _ref = C.CDLL(REF_PATH)
     return C.cast(_ref.hcubature_v, C.c_void_p).value
+
+
+def set_ulp_seed(seed: int) -> None:
+    """Sensitivity probe: a nonzero seed moves the integrands' exp, log, pow,
+    sin, cos and erf results by at most one unit in the last place (a stand-in
+    for another math library); 0 restores the exact functions."""
+    lib().orc_set_ulp_seed(seed)
 
 
 def _p(a):

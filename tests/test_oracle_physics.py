@@ -445,3 +445,37 @@ def test_integrands_vs_numpy_recomposition(tables, two_hod_model, stat, term, ty
     assert np.median(r) < 1e-11
     assert np.quantile(r, 0.99) < 1e-7
     assert np.array_equal(f == 0, g == 0)
+
+
+@pytest.mark.parametrize("stat,term", [(0, 1), (0, 3), (1, 2)])
+def test_ulp_probe_moves_integrands_by_rounding_only(oracle_model, stat, term):
+    """The sensitivity probe perturbs the elementary functions by at most one
+    unit in the last place: integrand values move at the rounding level, and
+    seed 0 restores bit-identical values.  Relative changes are large only for
+    masses just above the grid's lower end, where 1 + erf(.) of the HOD
+    occupations is a few units in the last place of 1 (integrand values below
+    1e-40 of the largest here)."""
+    import oracle as O
+    m = oracle_model
+    x = random_points(m, term, 2000, 300 + 10 * stat + term)
+    th = np.array([1.1, 3.0, 5.0]) * ARCMIN
+    f0 = m.integrand(stat, term, th, x, 1, 2)
+    try:
+        O.set_ulp_seed(7)
+        f1 = m.integrand(stat, term, th, x, 1, 2)
+    finally:
+        O.set_ulp_seed(0)
+    f2 = m.integrand(stat, term, th, x, 1, 2)
+    assert np.array_equal(f0, f2)
+    nz = np.abs(f0) > 1e-300
+    assert nz.sum() > 100
+    r = np.abs(f1[nz] - f0[nz]) / np.abs(f0[nz])
+    assert np.mean(r > 0) > 0.5          # the probe does change most values
+    assert np.median(r) < 1e-14          # ... at the rounding level
+    big = np.abs(f0[nz]) > 1e-12 * np.abs(f0).max()
+    assert big.sum() > 100
+    assert np.quantile(r[big], 0.9) < 1e-12
+    # the 3-halo bispectrum has ill-conditioned spots (cosine-rule angles,
+    # l3 -> 0) where one ulp in cos(phi) moves the value by up to ~1e-5
+    assert r[big].max() < 1e-4
+    assert np.abs(f1 - f0).max() < 1e-13 * np.abs(f0).max()
