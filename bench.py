@@ -60,6 +60,13 @@ def workload(name: str, hodset: int = 0):
                                               type1=1, type2=2, reltol=1e-4, maxeval=5000000))
         desc = {"workload": "NNMap 1h+2h+3h, 550 triplets 0.5-32 arcmin, tol 1e-4, maxEval 5e6",
                 "integrals": len(reqs), "nbins": 128, "hod": "fiducial", "types": "(1,2)"}
+    elif name == "nmapmap550":
+        for th in T.triplets_550():
+            for term in (1, 2, 3):
+                reqs.append(_lib.make_request(1, term, th * T.ARCMIN_IN_RAD, hodset=hodset,
+                                              type1=1, type2=2, reltol=1e-4, maxeval=5000000))
+        desc = {"workload": "NMapMap 1h+2h+3h, 550 triplets 0.5-32 arcmin, tol 1e-4, maxEval 5e6",
+                "integrals": len(reqs), "nbins": 128, "hod": "fiducial"}
     elif name == "nnmap4":
         for th in T.triplets_equal_4():
             reqs.append(_lib.make_request(0, 1, th * T.ARCMIN_IN_RAD, hodset=hodset, type1=1,
@@ -170,6 +177,48 @@ def serve(ctx, tables, n: int, rate: float, precision=_lib.FP64, seed: int = 4):
                     "grow"}
 
 
+def hod_sweep(ctx, n_hods: int = 256, pool_slots: int = 1 << 26, precision=_lib.FP64):
+    """The batched fit sweep: n_hods HOD vectors from the uniform prior x the
+    55 triplets (a_i, a_j, a_j) x NNMap 1h/2h/3h, every integral at tol 1e-4
+    and maxEval 5e6, submitted at once to one server whose region pool is
+    bounded (pool_slots slots, 249 B each): integrals are admitted as pool
+    segments free up and every admitted integral refines each round.  The HOD
+    sets' n_bar(z) are computed on the device inside the timed region."""
+    hods = T.hod_prior_draws(n_hods, seed=20220406)
+    trips = T.triplets_55()
+    t0 = time.perf_counter()
+    first = ctx.hodsets(hods, hods)
+    srv = _lib.Server(ctx, pool_slots=pool_slots, max_integrals=1 << 17, precision=precision)
+    nint, done, evals = 0, 0, 0
+    lat = []
+    try:
+        for h in range(n_hods):
+            reqs = [_lib.make_request(0, term, th * T.ARCMIN_IN_RAD, hodset=first + h, type1=1,
+                                      type2=2, reltol=1e-4, maxeval=5000000)
+                    for th in trips for term in (1, 2, 3)]
+            nint += len(reqs)
+            srv.submit(reqs, tag=h)
+        rounds = 0
+        while done < n_hods:
+            srv.step()
+            rounds += 1
+            for _, res in srv.poll():
+                done += 1
+                lat.append(time.perf_counter() - t0)
+                evals += sum(int(r.neval) for r in res)
+    finally:
+        srv.close()
+    span = time.perf_counter() - t0
+    return {"value": round(nint / span, 3), "unit": "integrals/s", "hod_sets": n_hods,
+            "integrals": nint, "seconds": round(span, 3), "rounds": rounds,
+            "evals": evals, "pool_slots": pool_slots,
+            "pool_gib": round(pool_slots * 249 / 2**30, 2),
+            "first_hod_done_s": round(min(lat), 3), "last_hod_done_s": round(max(lat), 3),
+            "workload": f"{n_hods} prior HOD vectors x 55 triplets (a_i, a_j, a_j) x NNMap "
+                        "1h/2h/3h, tol 1e-4, maxEval 5e6; device n_bar(z) of every HOD set "
+                        "inside the timed region; one server, bounded region pool"}
+
+
 def copy_probe(mib: int) -> int:
     """One elementwise float32 kernel that reads mib MiB and writes mib MiB: the
     known-size reference against which the HBM traffic counters are scaled."""
@@ -199,6 +248,10 @@ def main(argv=None):
                     help="requests in the served run (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=4.0,
                     help="mean Poisson arrival rate of the served run, requests/s")
+    ap.add_argument("--sweep-hods", type=int, default=256,
+                    help="HOD vectors in the batched fit sweep (0 skips it)")
+    ap.add_argument("--no-nmapmap", action="store_true",
+                    help="skip the NMapMap 550-triplet line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run an elementwise kernel that reads and writes this many MiB "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -348,6 +401,18 @@ def main(argv=None):
     line["nbar_sweep"] = {"hod_sets": 256, "integrals": 256 * 2 * tables.nbins,
                           "ms": round(nbar_ms, 3),
                           "integrals_per_s": round(256 * 2 * tables.nbins / (nbar_ms * 1e-3), 1)}
+    if not a.no_nmapmap and a.workload == "nnmap550":
+        # the same grid for the galaxy-shear-shear statistic, FP64 batched
+        nreqs, ndesc = workload("nmapmap550", sid)
+        ctx.eval_batch(nreqs)
+        t0 = time.perf_counter()
+        nres = ctx.eval_batch(nreqs)
+        dt = time.perf_counter() - t0
+        line["nmapmap550"] = {"value": round(len(nreqs) / dt, 3), "unit": "integrals/s",
+                              "ms_per_step": round(dt * 1e3, 3), "steps": 1, "warmup": 1,
+                              "evals": int(sum(int(r.neval) for r in nres)), **ndesc}
+    if a.sweep_hods > 0:
+        line["hod_sweep"] = hod_sweep(ctx, a.sweep_hods)
     if a.serve_requests > 0:
         line["served"] = serve(ctx, tables, a.serve_requests, a.serve_rate)
         line["latency_ms"]["served_p50"] = line["served"]["p50_ms"]

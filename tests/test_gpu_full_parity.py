@@ -1,14 +1,16 @@
-"""Full-size parity: the 1650-integral NNMap workload (550 aperture triplets
-x 1h/2h/3h, tol 1e-4, maxEval 5e6) on the device against the oracle's values
-for every integral (tests/golden/nnmap550_oracle.npz).
+"""Full-size parity: the 1650-integral NNMap and NMapMap workloads (550
+aperture triplets x 1h/2h/3h, tol 1e-4, maxEval 5e6) on the device against the
+oracle's values for every integral (tests/golden/nnmap550_oracle.npz,
+tests/golden/nmapmap550_oracle.npz).
 
 Every integral of this workload stops at the evaluation cap unconverged; its
 value depends on which regions were split, and a last-place difference in an
-integrand value can change that choice.  The 1-halo and 2-halo integrals
-refine identically (equal evaluation counts, values equal to rounding); the
-7-dimensional 3-halo integrals, whose error estimates are 3 % to several
-times their values, mostly do, and where they do not, the difference stays far
-inside the integral's own error estimate."""
+integrand value can change that choice.  Every NMapMap integral and the
+NNMap 1-halo and 2-halo integrals refine identically (equal evaluation counts,
+values equal to rounding); the 7-dimensional NNMap 3-halo integrals, whose
+error estimates are 3 % to several times their values, mostly do, and where
+they do not, the difference stays far inside the integral's own error
+estimate."""
 import os
 
 import numpy as np
@@ -18,20 +20,22 @@ from conftest import ROOT, T, say
 
 pytestmark = pytest.mark.gpu
 
-GOLD = os.path.join(ROOT, "tests", "golden", "nnmap550_oracle.npz")
+GOLD = {0: os.path.join(ROOT, "tests", "golden", "nnmap550_oracle.npz"),
+        1: os.path.join(ROOT, "tests", "golden", "nmapmap550_oracle.npz")}
 
 
-def test_nnmap550_against_oracle(tables):
+@pytest.mark.parametrize("stat", [0, 1])
+def test_550_against_oracle(tables, stat):
     from g3lhalo_amd import _lib
-    g = np.load(GOLD)
+    g = np.load(GOLD[stat])
     assert str(g["tables_checksum"]) == T.checksum(tables)
     c = _lib.Context(tables)
     try:
         sid = c.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
-        reqs = [_lib.make_request(0, term, th * T.ARCMIN_IN_RAD, hodset=sid, reltol=1e-4,
+        reqs = [_lib.make_request(stat, term, th * T.ARCMIN_IN_RAD, hodset=sid, reltol=1e-4,
                                   maxeval=5000000)
                 for th in T.triplets_550() for term in (1, 2, 3)]
-        say("full nnmap550 batch")
+        say(f"full 550-triplet batch, stat {stat}")
         out = c.eval_batch(reqs)
     finally:
         c.close()
@@ -45,6 +49,10 @@ def test_nnmap550_against_oracle(tables):
               f"max {dt.max():.2e} above 1e-4: {(dt > 1e-4).sum()}")
     assert np.array_equal(ne, g["neval"])
     assert d[0::3].max() < 1e-10 and d[1::3].max() < 1e-10
+    if stat == 1:
+        # NMapMap's 3-halo integrals refine identically too
+        assert d[2::3].max() < 1e-10
+        return
     d3 = d[2::3]
     assert np.median(d3) < 1e-6
     assert np.mean(d3 > 1e-4) < 0.05
