@@ -19,6 +19,7 @@ STATUS_NAMES = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ENOMEM_POOL",
                 5: "MAXEVAL_HIT", 6: "NONFINITE", 7: "ENODEV"}
 NNMAP, NMAPMAP = 0, 1
 FP64, MIXED = 0, 1
+LAUNCH_STREAM, LAUNCH_GRAPH = 0, 1
 
 _dp = C.POINTER(C.c_double)
 
@@ -90,6 +91,7 @@ SIGNATURES = {
     "g3l_last_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
     "g3l_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneReport)]),
     "g3l_set_variant": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "g3l_set_launch_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "g3l_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "g3l_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_size_t,
                                     C.POINTER(Result), C.c_uint64]),
@@ -240,6 +242,11 @@ class Context:
     def set_variant(self, kind, precision, variant):
         check("g3l_set_variant", load().g3l_set_variant(self.h, int(kind), int(precision),
                                                          int(variant)))
+
+    def set_launch_mode(self, mode):
+        """LAUNCH_GRAPH (default): one HIP graph launch per refinement round;
+        LAUNCH_STREAM: the same kernels launched one by one."""
+        check("g3l_set_launch_mode", load().g3l_set_launch_mode(self.h, int(mode)))
 
     def last_stats(self) -> BatchStats:
         s = BatchStats()

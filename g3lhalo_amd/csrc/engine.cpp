@@ -224,25 +224,61 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
   uint32_t rounds = 0;
   G3L_HIP(hipMemcpyAsync(hcnt, dcnt, (kNumKinds + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   G3L_HIP(hipStreamSynchronize(s));
-  for (;;) {
-    if (timed) G3L_HIP(hipEventRecord(c->ev[0], s));
+  // graph mode: the round's launches and the counter copy are one graph
+  // (every node records its events; they are read only when timed)
+  const bool use_graph = c->launch_mode == G3L_LAUNCH_GRAPH;
+  RoundGraph &graph = c->graph;
+  if (use_graph) {
+    // the executable graph is kept while the kinds, variants, precision,
+    // job count and buffers stay the same (repeated calls of one workload)
+    bool in_run[kNumKinds];
+    const int *var = c->variant[prec == G3L_MIXED];
+    std::vector<uint64_t> key = {(uint64_t)prec, nj, slots, (uint64_t)(uintptr_t)c->pool.p,
+                                 (uint64_t)(uintptr_t)c->lists.p, (uint64_t)(uintptr_t)c->jobs.p,
+                                 (uint64_t)(uintptr_t)c->state.p, (uint64_t)(uintptr_t)c->args.p,
+                                 (uint64_t)(uintptr_t)c->split.p,
+                                 (uint64_t)(uintptr_t)c->counters.p,
+                                 (uint64_t)(uintptr_t)c->hcounters.p};
     for (int k = 0; k < kNumKinds; ++k) {
-      kind_used[k] = hcnt[k] != 0;
+      in_run[k] = per_kind[k] != 0;
+      key.push_back(per_kind[k]);
+      key.push_back((uint64_t)var[k]);
+    }
+    if (!graph.exec || key != c->graph_key) {
+      c->graph_key.clear();
+      if ((rc = graph.build(in_run, prec, var, EL, SL, dcnt, kNumKinds + 1, hcnt, c->ev, c->kev)))
+        return rc;
+      c->graph_key = key;
+    }
+  }
+  for (;;) {
+    uint32_t cnt[kNumKinds];
+    for (int k = 0; k < kNumKinds; ++k) {
+      cnt[k] = (uint32_t)std::min<uint64_t>(hcnt[k], per_kind[k]);
+      kind_used[k] = cnt[k] != 0;
       if (!kind_used[k]) continue;
-      EL.list = lists[k];
-      EL.count = (uint32_t)std::min<uint64_t>(hcnt[k], per_kind[k]);
-      if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k], s));
-      G3L_HIP(launch_eval(k, prec, c->variant[prec == G3L_MIXED][k], EL, s));
-      if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k + 1], s));
       ++launches;
       ++kind_launches[k];
     }
-    if (timed) G3L_HIP(hipEventRecord(c->ev[1], s));
-    G3L_HIP(launch_zero(dcnt, kNumKinds + 1, s));
-    G3L_HIP(launch_select(SL, nj, s));
-    if (timed) G3L_HIP(hipEventRecord(c->ev[2], s));
-    G3L_HIP(hipMemcpyAsync(hcnt, dcnt, (kNumKinds + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           s));
+    if (use_graph) {
+      if ((rc = graph.launch(cnt, lists, nj, s))) return rc;
+    } else {
+      if (timed) G3L_HIP(hipEventRecord(c->ev[0], s));
+      for (int k = 0; k < kNumKinds; ++k) {
+        if (!kind_used[k]) continue;
+        EL.list = lists[k];
+        EL.count = cnt[k];
+        if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k], s));
+        G3L_HIP(launch_eval(k, prec, c->variant[prec == G3L_MIXED][k], EL, s));
+        if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k + 1], s));
+      }
+      if (timed) G3L_HIP(hipEventRecord(c->ev[1], s));
+      G3L_HIP(launch_zero(dcnt, kNumKinds + 1, s));
+      G3L_HIP(launch_select(SL, nj, s));
+      if (timed) G3L_HIP(hipEventRecord(c->ev[2], s));
+      G3L_HIP(hipMemcpyAsync(hcnt, dcnt, (kNumKinds + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             s));
+    }
     G3L_HIP(hipStreamSynchronize(s));
     if (timed) {
       float a = 0, b = 0;
@@ -888,6 +924,12 @@ int g3l_autotune(g3l_ctx *c, int32_t hodset, int32_t precision, g3l_tune_report 
       }
     }
   }
+  return G3L_OK;
+}
+
+int g3l_set_launch_mode(g3l_ctx *c, int32_t mode) {
+  if (!c || (mode != G3L_LAUNCH_STREAM && mode != G3L_LAUNCH_GRAPH)) return G3L_EINVAL;
+  c->launch_mode = mode;
   return G3L_OK;
 }
 

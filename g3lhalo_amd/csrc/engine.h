@@ -83,6 +83,37 @@ struct HodSet {
   int identical;
 };
 
+// One refinement round recorded as an instantiated HIP graph (round_graph.cpp):
+// an evaluation node per kind in use, each between two event records (kev),
+// the counter clear, the selection over the integrals in flight, an event
+// record, and the copy of ncounters round counters to pinned host memory.
+// launch() rewrites the evaluation and selection grids and arguments in the
+// executable graph, then launches it on s.
+struct RoundGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool in_use[kNumKinds] = {};
+  KernelRef eval_ref[kNumKinds] = {};
+  hipGraphNode_t eval_node[kNumKinds] = {};
+  EvalLaunch eval_args[kNumKinds] = {};
+  void *eval_params[kNumKinds][1] = {};
+  hipGraphNode_t select_node = nullptr;
+  SelectLaunch sel_args{};
+  void *sel_params[1] = {};
+  uint32_t *zero_ptr = nullptr;
+  int zero_n = 0;
+  void *zero_params[2] = {};
+  // ev[0..2]: round start, evaluations done, selection done; kev[2k], kev[2k+1]
+  // bracket kind k's evaluation
+  int build(const bool used[kNumKinds], int prec, const int variant[kNumKinds],
+            const EvalLaunch &eval_base, const SelectLaunch &sel_base, uint32_t *dcounters,
+            int ncounters, uint32_t *hcounters, const hipEvent_t *ev, const hipEvent_t *kev);
+  int launch(const uint32_t count[kNumKinds], EvalItem *const lists[kNumKinds], uint32_t nselect,
+             hipStream_t s);
+  void destroy();
+  ~RoundGraph() { destroy(); }
+};
+
 }  // namespace g3l
 
 struct g3l_ctx {
@@ -112,6 +143,11 @@ struct g3l_ctx {
   uint32_t pcap = 0;
   // launch variant (block shape) per precision and integrand kind
   int variant[2][kNumKinds] = {{0, 1, 2, 2, 1, 1, 2}, {0, 1, 2, 2, 1, 1, 2}};
+  // refinement rounds of the batched engine and the server: G3L_LAUNCH_GRAPH
+  // (one graph launch per round) or G3L_LAUNCH_STREAM (launches on the stream)
+  int launch_mode = G3L_LAUNCH_GRAPH;
+  g3l::RoundGraph graph;               // the batched engine's round graph
+  std::vector<uint64_t> graph_key;     // what it was built for
   hipEvent_t ev[4]{};
   hipEvent_t kev[2 * kNumKinds]{};
   g3l_batch_stats last{};

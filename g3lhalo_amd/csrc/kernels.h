@@ -73,6 +73,17 @@ struct GatherLaunch {
   uint32_t n;
 };
 
+// A kernel symbol with its block size and regions per block, for launches
+// recorded as graph nodes.
+struct KernelRef {
+  const void *fn;
+  unsigned threads;
+  unsigned group;
+};
+int eval_kernel_ref(int kind, int prec, int variant, KernelRef *out);
+KernelRef select_kernel_ref();   // k_select(SelectLaunch), one block per job
+KernelRef zero_kernel_ref();     // k_zero(uint32_t *p, int n)
+
 // variant: block shape, 0 .. eval_num_variants() - 1 (the same values for all)
 hipError_t launch_eval(int kind, int prec, int variant, const EvalLaunch &L, hipStream_t s);
 int eval_num_variants();

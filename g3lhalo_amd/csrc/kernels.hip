@@ -25,11 +25,12 @@ namespace g3l {
 template <int KIND, class P>
 __device__ __forceinline__ double integrand(const DevTables &T, const TermArgs &a,
                                             const double *x) {
+  using Q = typename EvalPolicy<KIND, P>::type;
   if constexpr (KIND == kNbar) {
-    return nbar_kernel<P>(T, a, x);
+    return nbar_kernel<Q>(T, a, x);
   } else {
-    DirectSrc<KIND, P> s(T, a, x);
-    return integrand_src<KIND, P>(T, a, s);
+    DirectSrc<KIND, Q> s(T, a, x);
+    return integrand_src<KIND, Q>(T, a, s);
   }
 }
 
@@ -215,6 +216,8 @@ struct Cache {
                 "seventeen (mass, z) classes");
   double l1[G][7], ap1[G][7], l2[G][7], ap2[G][7], cphi[G][7], z[G][7], w[G][7];
   double zpre[G][7], zfac[G][7];
+  // NNMap 3-halo: dwdz, 1/dwdz, g, p1, p2, 1/w per redshift (proj_3h)
+  double ztab[kind_of_kv(KIND) == kNN3 ? G : 1][7][6];
   int zr[G][7], zt[G][7];
   MassC m[S::NM][G][17];
   NfwC nf[cache_nfw<KIND>() ? S::NM : 1][cache_nfw<KIND>() ? G : 1][17];
@@ -308,8 +311,14 @@ struct CachedSrc {
   __device__ __forceinline__ double zpre() const {
     return C.zpre[g][kProj<D, bit(D - 1)>.cls[q]];
   }
-  __device__ __forceinline__ double zfac() const {
-    return C.zfac[g][kProj<D, bit(D - 1)>.cls[q]];
+  __device__ __forceinline__ double proj(double B) const {
+    const int cz = kProj<D, bit(D - 1)>.cls[q];
+    if constexpr (kind_of_kv(KIND) == kNN3) {
+      const double *t = C.ztab[g][cz];
+      return proj_3h(B, t[0], t[1], t[2], t[3], t[4], C.w[g][cz], t[5], C.z[g][cz]);
+    } else {
+      return C.zfac[g][cz] * B;
+    }
   }
 };
 
@@ -342,16 +351,16 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     const int which = r / (G * 7), rr = r - which * G * 7, g = rr / 7, c = rr - g * 7;
     if (g >= ng || St.slot[g] == 0xffffffffu) return;
     const TermArgs &a = St.args[g];
+    // l1 and l2 items share one wave: one code path with selected operands
+    rule_point<D>(which == 0 ? kProj<D, kL1>.rep[c] : kProj<D, kL2>.rep[c], St.c[g], St.h[g], x);
+    const double l = P::l(which == 0 ? x[0] : x[1]);
+    const double ap = P::ap((which == 0 ? a.t1 : a.t2) * l);
     if (which == 0) {
-      rule_point<D>(kProj<D, kL1>.rep[c], St.c[g], St.h[g], x);
-      const double l = P::l(x[0]);
       C.l1[g][c] = l;
-      C.ap1[g][c] = P::ap(a.t1 * l);
+      C.ap1[g][c] = ap;
     } else {
-      rule_point<D>(kProj<D, kL2>.rep[c], St.c[g], St.h[g], x);
-      const double l = P::l(x[1]);
       C.l2[g][c] = l;
-      C.ap2[g][c] = P::ap(a.t2 * l);
+      C.ap2[g][c] = ap;
     }
   } else if (i < Lay::Z) {
     const int r = i - Lay::PHI;
@@ -374,6 +383,15 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     C.zt[g][c] = zix_trunc(T, z);
     C.w[g][c] = w;
     zfactors<kind_of_kv(KIND)>(T, St.args[g], zr, z, w, C.zpre[g][c], C.zfac[g][c]);
+    if constexpr (kind_of_kv(KIND) == kNN3) {
+      double *t = C.ztab[g][c];
+      t[0] = T.dwdz[zr];
+      t[1] = 1. / t[0];
+      t[2] = T.g[zr];
+      t[3] = T.p1[zr];
+      t[4] = T.p2[zr];
+      t[5] = 1. / w;
+    }
   } else if (i < Lay::PK) {
     const int r = i - Lay::MZ;
     if (r >= S::NM * G * 17) return;
@@ -387,21 +405,31 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     else rep = kProj<D, bit(S::mdim[S::NM > 2 ? 2 : 0]) | bit(D - 1)>.rep[c];
     rule_point<D>(rep, St.c[g], St.h[g], x);
     const int zt = zix_trunc(T, x[D - 1]);
-    const Mass M = make_mass(T, x[md], zt);
+    const Mass M = make_mass<P>(T, x[md], zt);
     MassC &e = C.m[slot][g][c];
     e.m = M.m;
     e.L = M.L;
     e.r200 = M.r200;
     e.ir = M.ir;
     e.it = M.it;
-    const Occ o1 = occ1<P>(a, M);
-    e.nc1 = o1.nc;
-    e.ns1 = o1.ns;
-    if constexpr (S::two_hods) {
-      const Occ o2 = occ2<P>(a, M);
-      e.nc2 = o2.nc;
-      e.ns2 = o2.ns;
-      if (slot == S::nss_slot) e.nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+    if constexpr (kind_of_kv(KIND) == kNN3) {
+      // the NNMap 3-halo term needs the first HOD at m1 and the second at m2
+      // only: one occupation per item, the HOD selected by the mass slot
+      const bool second = slot == 1;
+      const Occ o = P::occ(second ? a.h2 : a.h1, second ? a.lth2 : a.lth1,
+                           second ? a.lmp2 : a.lmp1, M);
+      e.nc1 = e.nc2 = o.nc;
+      e.ns1 = e.ns2 = o.ns;
+    } else {
+      const Occ o1 = occ1<P>(a, M);
+      e.nc1 = o1.nc;
+      e.ns1 = o1.ns;
+      if constexpr (S::two_hods) {
+        const Occ o2 = occ2<P>(a, M);
+        e.nc2 = o2.nc;
+        e.ns2 = o2.ns;
+        if (slot == S::nss_slot) e.nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+      }
     }
     if constexpr (cache_nfw<KIND>()) {
       NfwC &en = C.nf[slot][g][c];
@@ -568,22 +596,23 @@ __global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
   __shared__ Stage<D, G> St;
   __shared__ Cache<KIND> C;
   __shared__ double s_f[G * NP];
+  using Q = typename EvalPolicy<kind_of_kv(KIND), P>::type;
   if (blockIdx.x * G >= L.count) return;
   const int ng = stage_regions<D, G, NT>(L, St);
   const int tid = threadIdx.x;
   const DevTables T = L.T;
-  pass1<KIND, P>(T, St, C, tid, ng);
-  if constexpr (P1Layout<KIND>::END > NT) pass1<KIND, P>(T, St, C, tid + NT, ng);
+  pass1<KIND, Q>(T, St, C, tid, ng);
+  if constexpr (P1Layout<KIND>::END > NT) pass1<KIND, Q>(T, St, C, tid + NT, ng);
   __syncthreads();
-  pass2<KIND, P>(T, St, C, tid, ng);
-  if constexpr (P2 > NT) pass2<KIND, P>(T, St, C, tid + NT, ng);
-  if constexpr (P2 > 2 * NT) pass2<KIND, P>(T, St, C, tid + 2 * NT, ng);
+  pass2<KIND, Q>(T, St, C, tid, ng);
+  if constexpr (P2 > NT) pass2<KIND, Q>(T, St, C, tid + NT, ng);
+  if constexpr (P2 > 2 * NT) pass2<KIND, Q>(T, St, C, tid + 2 * NT, ng);
   __syncthreads();
   if (tid < ng * NP) {
     const int g = tid / NP, q = tid - g * NP;
     if (St.slot[g] != 0xffffffffu) {
-      CachedSrc<KIND, P> src(C, St.args[g], g, q);
-      s_f[tid] = integrand_src<kind_of_kv(KIND), P>(T, St.args[g], src);
+      CachedSrc<KIND, Q> src(C, St.args[g], g, q);
+      s_f[tid] = integrand_src<kind_of_kv(KIND), Q>(T, St.args[g], src);
     }
   }
   __syncthreads();
@@ -1037,6 +1066,50 @@ hipError_t launch_eval(int kind, int prec, int variant, const EvalLaunch &L, hip
 }
 
 int eval_num_variants() { return kNumVariants; }
+
+// Kernel symbols and block shapes, for launches recorded in a graph.
+template <int KV, class P>
+static KernelRef eval_ref_kind() {
+  using Gr = KGroup<KV>;
+  KernelRef r;
+  if constexpr (kind_of_kv(KV) == kNbar)
+    r.fn = reinterpret_cast<const void *>(&k_eval_direct<KV, P>);
+  else
+    r.fn = reinterpret_cast<const void *>(&k_eval<KV, P>);
+  r.threads = (unsigned)Gr::NT;
+  r.group = (unsigned)Gr::G;
+  return r;
+}
+
+template <int V, class P>
+static int eval_ref_variant(int kind, KernelRef *out) {
+  switch (kind) {
+    case kNN1: *out = eval_ref_kind<make_kv(kNN1, V), P>(); return 0;
+    case kNN2: *out = eval_ref_kind<make_kv(kNN2, V), P>(); return 0;
+    case kNN3: *out = eval_ref_kind<make_kv(kNN3, V), P>(); return 0;
+    case kNM1: *out = eval_ref_kind<make_kv(kNM1, V), P>(); return 0;
+    case kNM2: *out = eval_ref_kind<make_kv(kNM2, V), P>(); return 0;
+    case kNM3: *out = eval_ref_kind<make_kv(kNM3, V), P>(); return 0;
+    default: return -1;
+  }
+}
+
+int eval_kernel_ref(int kind, int prec, int variant, KernelRef *out) {
+  if (kind == kNbar) {
+    *out = eval_ref_kind<kNbar, P64>();
+    return 0;
+  }
+  if (variant < 0 || variant >= kNumVariants) return -1;
+  if (prec == 0) {
+    if (variant == 2) return eval_ref_variant<2, P64>(kind, out);
+    return variant ? eval_ref_variant<1, P64>(kind, out) : eval_ref_variant<0, P64>(kind, out);
+  }
+  if (variant == 2) return eval_ref_variant<2, PMix>(kind, out);
+  return variant ? eval_ref_variant<1, PMix>(kind, out) : eval_ref_variant<0, PMix>(kind, out);
+}
+
+KernelRef select_kernel_ref() { return KernelRef{reinterpret_cast<const void *>(&k_select), 256u, 1u}; }
+KernelRef zero_kernel_ref() { return KernelRef{reinterpret_cast<const void *>(&k_zero), 256u, 1u}; }
 
 int eval_group_size(int kind, int variant) {
   const int d = kind_dim(kind);

@@ -12,11 +12,15 @@
 //          index chains, table values, products, the NFW normalisation).
 //
 // Deviations of P64 from the reference that change only the last bits:
-// Si/Ci for x >= 4 by Horner's rule in 1/x^2; u_NFW with both arguments >= 4
-// by the cancellation-free identity noted at P64::u; the HOD logarithms and
-// powers from log(m/mmin); cbrt for pow(., 1/3); exp10 for the 10^x of
-// l1, l2; shared sin/cos of one argument; sub-expressions the reference
-// evaluates repeatedly with equal arguments evaluated once.
+// u_NFW by the cancellation-free branch identities noted at P64::u, with
+// Si/Ci's polynomials summed over precomputed powers and sin/cos by our own
+// Cody-Waite/Taylor routine; the HOD logarithms and powers from
+// log(m/mmin); cbrt for pow(., 1/3); exp10 for the 10^x of l1, l2; the
+// redshift-only projection factor multiplied into B in one step; shared
+// sin/cos of one argument; sub-expressions the reference evaluates
+// repeatedly with equal arguments evaluated once.  The NNMap 3-halo term,
+// whose integrals are sensitive to last-bit differences, keeps the
+// reference's HOD forms (P64R) and its order for the projection factor.
 // Masses keep pow(10, x) and the index chains keep the reference's exact
 // order: region centres sit exactly on rounding ties of the nearest-index
 // look-ups (the root centre maps to mass index 63.5 at 128 bins).
@@ -156,13 +160,14 @@ struct Mass {
   int ir, it;    // round and int() indices
   double r200;
 };
+template <class P>
 G3L_HD Mass make_mass(const DevTables &T, double x, int zt) {
   Mass M;
   M.m = pow(10.0, x);
   M.L = log(M.m / T.mmin);
   M.ir = clampi((int)round(M.L * T.nb / T.lmr), T.nb - 1);
   M.it = clampi((int)(M.L * T.nb / T.lmr), T.nb - 1);
-  M.r200 = cbrt(0.239 * M.m / T.rho[zt] / 200);
+  M.r200 = P::cbrt3(0.239 * M.m / T.rho[zt] / 200);
   return M;
 }
 
@@ -290,6 +295,8 @@ G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
 struct P64 {
   static constexpr bool kMixed = false;
   G3L_HD static double l(double x) { return exp10(x); }
+  // x^(1/3) of r_200
+  G3L_HD static double cbrt3(double x) { return cbrt(x); }
 
   // Ncen and Nsat share log(m/M_th)/sigma/sqrt2 and its erf (src/HOD.cu:35-76);
   // the logarithms come from L = log(m/mmin): log(m/M_th) = L + log(mmin/M_th),
@@ -352,6 +359,37 @@ struct P64 {
   // U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42).
   G3L_HD static double ap(double eta) {
     return 0.5 * eta * eta * exp(-0.5 * eta * eta);
+  }
+};
+
+// FP64 with the reference's own expressions for 10^x of l1, l2 and the r_200
+// cube root (pow), the HOD occupations and NsatNsat's power
+// (src/HOD.cu:35-93: log(m/M_th) and pow(m/M', beta) of the mass itself;
+// Nsat's repeated erf term evaluated once).  The NNMap 3-halo term uses it:
+// its squeezed bispectrum configurations (k3 -> 0, where the cosine-rule
+// angles lose most of their digits) make that term's integrals sensitive to
+// last-bit differences in any factor, and these forms, together with the
+// projection factor in the reference's order, bring its values closest to
+// the reference's (about half as many integrals move by more than 1e-4).
+// The other terms refine identically either way and keep P64's cheaper
+// forms.
+struct P64R : P64 {
+  // 10^x of l1, l2 and x^(1/3) of r_200 as the reference's pow
+  // (src/NNMap_Model.cu:986-991, src/generalFunctions.cu:36)
+  G3L_HD static double l(double x) { return pow(10.0, x); }
+  G3L_HD static double cbrt3(double x) { return pow(x, 1. / 3.); }
+  G3L_HD static Occ occ(const DevHod &h, double lth, double lmp, const Mass &M) {
+    (void)lth;
+    (void)lmp;
+    double e = 1 + erf(log(M.m / h.mth) / h.sigma / 1.414213562);
+    Occ o;
+    o.nc = 0.5 * h.alpha * e;
+    o.ns = 0.5 * e * pow(M.m / h.mprime, h.beta);
+    return o;
+  }
+  G3L_HD static double mpow(const Mass &M, double eps, double lmmin) {
+    (void)lmmin;
+    return pow(M.m, eps);
   }
 };
 
@@ -469,6 +507,7 @@ G3L_HD float x_minus_sin(float x, float sx) {
 struct PMix {
   static constexpr bool kMixed = true;
   G3L_HD static double l(double x) { return exp10(x); }
+  G3L_HD static double cbrt3(double x) { return cbrt(x); }
 
   // log(m/M_th) = L + log(mmin/M_th); 1 + erf(X) = erfc(-X), for X < -1/2
   // as exp(-X^2) (split precision) times the smooth erfcx(-X) in FP32;
@@ -684,7 +723,8 @@ G3L_HD double f_of(const TermArgs &a, int fsel) {
 // Redshift-only factors of the integrands, per redshift index zi:
 //   zpre = 1/n1/n2/rho (NNMap), 1/n/rho/rho (NMapMap), the leading factor of B;
 //   zfac = g p1 p2 (1+z) / dwdz / w^3 (NNMap), g^2 p1 (1+z)^2 / dwdz / w^2
-//   (NMapMap), the projection factor b = B zfac.
+//   (NMapMap), the projection factor b = zfac B (the NNMap 3-halo term
+//   applies it in the reference's order instead, proj_3h).
 template <int KIND>
 G3L_HD void zfactors(const DevTables &T, const TermArgs &a, int zi, double z, double w,
                      double &zpre, double &zfac) {
@@ -698,6 +738,45 @@ G3L_HD void zfactors(const DevTables &T, const TermArgs &a, int zi, double z, do
   }
 }
 
+// The evaluation policy of an integrand: P64R for the NNMap 3-halo term and
+// the n_bar integrand in FP64, the requested policy otherwise.
+template <int KIND, class P>
+struct EvalPolicy {
+  using type = P;
+};
+template <>
+struct EvalPolicy<kNN3, P64> {
+  using type = P64R;
+};
+// n_bar(z), a factor of every term, with the reference's HOD expressions too
+template <>
+struct EvalPolicy<kNbar, P64> {
+  using type = P64R;
+};
+
+// a / d correctly rounded from rd = RN(1/d): q = RN(a rd), then one FMA
+// residual correction (Markstein's theorem: with rd the correctly rounded
+// reciprocal the corrected quotient is RN(a / d) for normal operands, no
+// overflow or underflow).  Three instructions instead of the IEEE division
+// sequence, for divisors that repeat over many points.
+G3L_HD double div_rn(double a, double d, double rd) {
+  const double q = a * rd;
+  return fma(fma(-d, q, a), rd, q);
+}
+
+// The NNMap 3-halo projection b = B / dwdz g p1 p2 / w^3 (1+z) in the
+// reference's order (src/NNMap_Model.cu:1031-1033), from the divisors and
+// their correctly rounded reciprocals; equal to the IEEE-division form.  The
+// other terms use the factor zfac of zfactors: b = zfac B.
+G3L_HD double proj_3h(double B, double dwdz, double rdwdz, double g, double p1, double p2,
+                      double w, double rw, double z) {
+  double b = div_rn(B, dwdz, rdwdz) * g * p1 * p2;
+  b = div_rn(b, w, rw);
+  b = div_rn(b, w, rw);
+  b = div_rn(b, w, rw);
+  return b * (1. + z);
+}
+
 // Direct source: every quantity computed at the point itself.
 template <int KIND, class P>
 struct DirectSrc {
@@ -709,7 +788,7 @@ struct DirectSrc {
   G3L_HD DirectSrc(const DevTables &T_, const TermArgs &a_, const double *x) : T(T_), a(a_) {
     G = make_geo<P>(T, x, S::D);
 #pragma unroll
-    for (int i = 0; i < S::NM; ++i) M[i] = make_mass(T, x[S::mdim[i]], G.zt);
+    for (int i = 0; i < S::NM; ++i) M[i] = make_mass<P>(T, x[S::mdim[i]], G.zt);
   }
   G3L_HD const Geo &geo() const { return G; }
   G3L_HD Mass mass(int i) const { return M[i]; }
@@ -735,10 +814,15 @@ struct DirectSrc {
     zfactors<KIND>(T, a, G.zr, G.z, G.w, p, f);
     return p;
   }
-  G3L_HD double zfac() const {
-    double p, f;
-    zfactors<KIND>(T, a, G.zr, G.z, G.w, p, f);
-    return f;
+  G3L_HD double proj(double B) const {
+    const int zi = G.zr;
+    if constexpr (KIND == kNN3) {
+      return B / T.dwdz[zi] * T.g[zi] * T.p1[zi] * T.p2[zi] / G.w / G.w / G.w * (1. + G.z);
+    } else {
+      double p, f;
+      zfactors<KIND>(T, a, zi, G.z, G.w, p, f);
+      return f * B;
+    }
   }
 };
 
@@ -759,7 +843,7 @@ G3L_HD double nnmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   double m = M.m, l1 = G.l1, l2 = G.l2;
   double B = s.zpre() * T.hmf[zi * T.nb + M.ir] * m * u3;
   B *= o1.nc * o2.ns * u2 + o2.nc * o1.ns * u1 + nss * u1 * u2;
-  double b = B * s.zfac();
+  double b = s.proj(B);
   return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b;
 }
 
@@ -791,7 +875,7 @@ G3L_HD double nnmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
              T.hmf[zi * nb + M2.ir] * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              (m1 * u_k3_m1 * Ggg * P3 + m2 * u_k3_m2 * G_k1_m1_h1 * G_k2_m2_h2 * P1 +
               m2 * u_k3_m2 * G_k2_m1_h2 * G_k1_m2_h1 * P2);
-  double b = s.zfac() * B;
+  double b = s.proj(B);
   return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b;
 }
 
@@ -810,7 +894,7 @@ G3L_HD double nnmap_3h(const DevTables &T, const TermArgs &a, const S &s) {
   double B = s.zpre() * T.hmf[zi * nb + M1.ir] *
              T.hmf[zi * nb + M2.ir] * T.hmf[zi * nb + M3.ir] * m3 * u3 * G1 * G2 * s.bl() *
              T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] * T.bh[zi * nb + M3.ir];
-  double b = B * s.zfac();
+  double b = s.proj(B);
   return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b;
 }
 
@@ -827,7 +911,7 @@ G3L_HD double nmapmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   double m = M.m, l1 = G.l1, l2 = G.l2;
   double B = s.zpre() * T.hmf[zi * T.nb + M.ir] * m * m * (o.nc + o.ns * uf) *
              s.template u<1>() * P::u(n1, G.k3);
-  double b = B * s.zfac();
+  double b = s.proj(B);
   return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b / nbar;
 }
 
@@ -850,7 +934,7 @@ G3L_HD double nmapmap_2h(const DevTables &T, const TermArgs &a, const S &s) {
              (m1 * m2 * u_k2_m1 * u_k3_m2 * (o2.nc + o2.ns * u_k1_m2_f) * P2 +
               m1 * m2 * u_k3_m1 * u_k2_m2 * (o2.nc + o2.ns * u_k1_m2_f) * P3 +
               m2 * m2 * u_k2_m1 * u_k3_m2 * (o1.nc + o1.ns * u_k1_m1_f) * P1);
-  double b = B * s.zfac();
+  double b = s.proj(B);
   return l1 * l2 * m1 * m2 * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b / nbar;
 }
 
@@ -871,7 +955,7 @@ G3L_HD double nmapmap_3h(const DevTables &T, const TermArgs &a, const S &s) {
              T.hmf[zi * nb + M3.ir] * T.bh[zi * nb + M1.ir] * T.bh[zi * nb + M2.ir] *
              T.bh[zi * nb + M3.ir] * m1 * m2 * u_k2_m1 * u_k3_m2 * (o.nc + o.ns * u_k1_m2_f) *
              s.bl();
-  double b = B * s.zfac();
+  double b = s.proj(B);
   return l1 * l2 * m1 * m2 * m3 * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b / nbar;
 }
 
@@ -899,7 +983,7 @@ G3L_HD double nbar_kernel(const DevTables &T, const TermArgs &a,
                                               const double *x) {
   double z = a.t1;
   int zr = zix_round(T, z);
-  Mass M = make_mass(T, x[0], zix_trunc(T, z));
+  Mass M = make_mass<P>(T, x[0], zix_trunc(T, z));
   Occ o = occ1<P>(a, M);
   return M.m * T.hmf[zr * T.nb + M.ir] * (o.nc + o.ns);
 }

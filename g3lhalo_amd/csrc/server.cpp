@@ -55,6 +55,10 @@ struct g3l_server {
   std::deque<uint64_t> completed;
   uint32_t fill[kNumKinds] = {};  // evaluation-list entries for the next round
   uint64_t rounds = 0;
+  // the round as a graph (launch mode G3L_LAUNCH_GRAPH), built at the first
+  // step and again whenever the context's launch variants change
+  RoundGraph graph;
+  int graph_variant[kNumKinds] = {};
 };
 
 namespace {
@@ -294,13 +298,6 @@ int g3l_server_step(g3l_server *s, g3l_server_status *status) {
   EL.split = s->split.as<uint8_t>();
   EL.stride = s->slots;
   EL.njobs = s->max_jobs;
-  for (int k = 0; k < kNumKinds; ++k) {
-    if (!s->fill[k]) continue;
-    EL.list = s->lists_k[k];
-    EL.count = s->fill[k];
-    G3L_HIP(launch_eval(k, s->precision, c->variant[s->precision == G3L_MIXED][k], EL, st));
-  }
-  G3L_HIP(launch_zero(dcnt, kNumKinds + 2, st));
   SelectLaunch SL{};
   SL.jobs = s->jobs.as<JobDesc>();
   SL.state = s->state.as<JobState>();
@@ -314,9 +311,29 @@ int g3l_server_step(g3l_server *s, g3l_server_status *status) {
   SL.counters = dcnt;
   SL.active = s->active.as<uint32_t>();
   SL.done = s->done.as<uint32_t>();
-  G3L_HIP(launch_select(SL, na, st));
   uint32_t *hc = s->hcounters.as<uint32_t>();
-  G3L_HIP(hipMemcpyAsync(hc, dcnt, (kNumKinds + 2) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  const int *var = c->variant[s->precision == G3L_MIXED];
+  if (c->launch_mode == G3L_LAUNCH_GRAPH) {
+    if (!s->graph.exec || !std::equal(var, var + kNumKinds, s->graph_variant)) {
+      bool in_use[kNumKinds];
+      for (int k = 0; k < kNumKinds; ++k) in_use[k] = k != kNbar;  // the six terms
+      if ((rc = s->graph.build(in_use, s->precision, var, EL, SL, dcnt, kNumKinds + 2, hc, c->ev,
+                               c->kev)))
+        return rc;
+      std::copy(var, var + kNumKinds, s->graph_variant);
+    }
+    if ((rc = s->graph.launch(s->fill, s->lists_k, na, st))) return rc;
+  } else {
+    for (int k = 0; k < kNumKinds; ++k) {
+      if (!s->fill[k]) continue;
+      EL.list = s->lists_k[k];
+      EL.count = s->fill[k];
+      G3L_HIP(launch_eval(k, s->precision, var[k], EL, st));
+    }
+    G3L_HIP(launch_zero(dcnt, kNumKinds + 2, st));
+    G3L_HIP(launch_select(SL, na, st));
+    G3L_HIP(hipMemcpyAsync(hc, dcnt, (kNumKinds + 2) * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  }
   G3L_HIP(hipStreamSynchronize(st));
   ++s->rounds;
   for (int k = 0; k < kNumKinds; ++k) s->fill[k] = hc[k];

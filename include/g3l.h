@@ -164,6 +164,16 @@ int g3l_autotune(g3l_ctx *ctx, int32_t hodset, int32_t precision, g3l_tune_repor
 /* Sets one kind's variant directly (kind 1-6). */
 int g3l_set_variant(g3l_ctx *ctx, int32_t kind, int32_t precision, int32_t variant);
 
+/* How the refinement rounds of g3l_eval_batch and of the servers made on
+ * this context reach the device.  No counterpart in the reference (its host
+ * loop issues one synchronous launch per batch, src/NNMap_Model.cu:596-631).
+ * G3L_LAUNCH_GRAPH (the default): each round's evaluation, counter-clear,
+ * selection and counter-copy operations are one instantiated HIP graph whose
+ * kernel nodes are re-sized per round; G3L_LAUNCH_STREAM: the same kernels
+ * launched one by one on the context's stream.  Results are identical. */
+typedef enum g3l_launch_mode { G3L_LAUNCH_STREAM = 0, G3L_LAUNCH_GRAPH = 1 } g3l_launch_mode;
+int g3l_set_launch_mode(g3l_ctx *ctx, int32_t mode);
+
 /* Version string of this build. */
 const char *g3l_version(void);
 

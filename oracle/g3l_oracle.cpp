@@ -50,7 +50,13 @@ const double kMpcKm = 3.086e19;
  * in for a different, equally accurate math library.  An integral that stops
  * at the evaluation cap unconverged can refine differently under such a
  * change; the spread it causes is the scale against which a second
- * implementation's values are judged.  Seed 0 (the default) changes nothing. */
+ * implementation's values are judged.  Seed 0 (the default) changes nothing.
+ * The nearest-index chains (the masses 10^x and the logarithms that pick a
+ * table row) are left exact: region centres sit exactly on rounding ties of
+ * the mass index (the root centre maps to 63.5 at 128 bins), where a one-ulp
+ * change moves every point of every region unsplit in that mass, so an
+ * implementation that matches the reference must reproduce those chains bit
+ * for bit rather than to within an ulp. */
 uint64_t g_ulp_seed = 0;
 
 inline double ulp_jitter(double r, double a, double b) {
@@ -535,10 +541,10 @@ struct Model {
    * int() truncation inside u_NFW and r_200 (src/generalFunctions.cu:11-12,35). */
   inline int zix_round(double z) const { return (int)std::round((z - zmin) * nb / (zmax - zmin)); }
   inline int mix_round(double m) const {
-    return (int)std::round(Jlog(m / mmin) * nb / Jlog(mmax / mmin));
+    return (int)std::round(std::log(m / mmin) * nb / std::log(mmax / mmin));
   }
   inline int kix_round(double k) const {
-    return (int)std::round(Jlog(k / kmin) * nb / Jlog(kmax / kmin));
+    return (int)std::round(std::log(k / kmin) * nb / std::log(kmax / kmin));
   }
 
   /* P(k) look-up (src/NNMap_Model.cu:819-839): below kmin linear in k; no
@@ -557,7 +563,7 @@ struct Model {
   /* Truncated-NFW Fourier profile (src/generalFunctions.cu:6-30). */
   double unfw(double k, double m, double z, double f) const {
     int zi = int((z - zmin) * nb / (zmax - zmin));
-    int mi = int(Jlog(m / mmin) * nb / Jlog(mmax / mmin));
+    int mi = int(std::log(m / mmin) * nb / std::log(mmax / mmin));
     double c = f * conc[(size_t)zi * nb + mi];
     double arg1 = k * r200(m, z) / c;
     double arg2 = arg1 * (1 + c);
@@ -771,31 +777,31 @@ int term_cb(unsigned ndim, size_t npts, const double *x, void *fd, unsigned fdim
     double r = 0;
     switch (key) {
       case 0: { /* n_bar */
-        double m = Jpow(10, p[0]);
+        double m = std::pow(10, p[0]);
         r = T.M->k_nbar(m, T.z, *T.h1);
         break;
       }
       case 1:
-        r = nnm_1h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]), p[4]);
+        r = nnm_1h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], std::pow(10, p[3]), p[4]);
         break;
       case 2:
-        r = nnm_2h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]),
-                   Jpow(10, p[4]), p[5]);
+        r = nnm_2h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], std::pow(10, p[3]),
+                   std::pow(10, p[4]), p[5]);
         break;
       case 3:
-        r = nnm_3h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]),
-                   Jpow(10, p[4]), Jpow(10, p[5]), p[6]);
+        r = nnm_3h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], std::pow(10, p[3]),
+                   std::pow(10, p[4]), std::pow(10, p[5]), p[6]);
         break;
       case 5:
-        r = nmm_1h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]), p[4]);
+        r = nmm_1h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], std::pow(10, p[3]), p[4]);
         break;
       case 6:
-        r = nmm_2h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]),
-                   Jpow(10, p[4]), p[5]);
+        r = nmm_2h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], std::pow(10, p[3]),
+                   std::pow(10, p[4]), p[5]);
         break;
       case 7:
-        r = nmm_3h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], Jpow(10, p[3]),
-                   Jpow(10, p[4]), Jpow(10, p[5]), p[6]);
+        r = nmm_3h(T, Jpow(10, p[0]), Jpow(10, p[1]), p[2], std::pow(10, p[3]),
+                   std::pow(10, p[4]), std::pow(10, p[5]), p[6]);
         break;
       default:
         r = 0;

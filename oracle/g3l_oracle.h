@@ -110,7 +110,9 @@ uint32_t orc_points_per_region(unsigned dim);
 
 /* Sensitivity probe (tests only): a nonzero seed moves every exp, log, pow,
  * sin, cos and erf result inside the integrands by at most one unit in the
- * last place, by a hash of the argument; 0 restores the exact functions. */
+ * last place, by a hash of the argument, except the nearest-index chains
+ * (masses 10^x and the logarithms that pick a table row); 0 restores the
+ * exact functions. */
 void orc_set_ulp_seed(uint64_t seed);
 
 #ifdef __cplusplus

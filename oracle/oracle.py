@@ -96,7 +96,9 @@ def reference_hcubature():
 def set_ulp_seed(seed: int) -> None:
     """Sensitivity probe: a nonzero seed moves the integrands' exp, log, pow,
     sin, cos and erf results by at most one unit in the last place (a stand-in
-    for another math library); 0 restores the exact functions."""
+    for another math library), except the nearest-index chains (masses 10^x
+    and the table-row logarithms, whose region-centre arguments sit on
+    rounding ties); 0 restores the exact functions."""
     lib().orc_set_ulp_seed(seed)
 
 

@@ -3,7 +3,9 @@ the 1650-integral NNMap workload (550 aperture triplets, fiducial HOD, types
 (1,2), tol 1e-4, maxEval 5e6) recomputed with its sensitivity probe on: every
 exp, log, pow, sin, cos and erf result inside the integrand moved by at most
 one unit in the last place (seeds 1 and 2), which is what a second, equally
-accurate math library does to the same algorithm.
+accurate math library does to the same algorithm -- except the
+nearest-index chains, which any implementation that matches the reference
+has to reproduce bit for bit.
 
 Every 3-halo integral of this workload stops at the evaluation cap
 unconverged, so its value depends on which near-tied regions were split.  The

@@ -8,9 +8,11 @@ value depends on which regions were split, and a last-place difference in an
 integrand value can change that choice.  Every NMapMap integral and the
 NNMap 1-halo and 2-halo integrals refine identically (equal evaluation counts,
 values equal to rounding); the 7-dimensional NNMap 3-halo integrals, whose
-error estimates are 3 % to several times their values, mostly do, and where
-they do not, the difference stays far inside the integral's own error
-estimate."""
+error estimates are 3 % to several times their values, mostly do.  Where they
+do not, the difference is of the size the oracle itself shows when its math
+library moves by one unit in the last place (tests/golden/
+nnmap550_oracle_ulp.npz: the same few integrals move, by up to ~5e-4), and it
+stays far inside the integral's own error estimate."""
 import os
 
 import numpy as np
@@ -22,6 +24,7 @@ pytestmark = pytest.mark.gpu
 
 GOLD = {0: os.path.join(ROOT, "tests", "golden", "nnmap550_oracle.npz"),
         1: os.path.join(ROOT, "tests", "golden", "nmapmap550_oracle.npz")}
+ULP = os.path.join(ROOT, "tests", "golden", "nnmap550_oracle_ulp.npz")
 
 
 @pytest.mark.parametrize("stat", [0, 1])
@@ -54,7 +57,15 @@ def test_550_against_oracle(tables, stat):
         assert d[2::3].max() < 1e-10
         return
     d3 = d[2::3]
-    assert np.median(d3) < 1e-6
-    assert np.mean(d3 > 1e-4) < 0.05
+    # the scale set by the reference's algorithm itself: the oracle's 3-halo
+    # values under one-ulp perturbations of its math library (two seeds)
+    u = np.load(ULP)
+    assert str(u["tables_checksum"]) == T.checksum(tables)
+    du = np.abs(u["value"] - g["value"][2::3]) / np.abs(g["value"][2::3])
+    n_ulp = int((du > 1e-4).sum(axis=1).max())
+    print(f"NNMap 3h: device above 1e-4: {(d3 > 1e-4).sum()}, max {d3.max():.2e}; oracle under "
+          f"1-ulp perturbation: above 1e-4 at most {n_ulp}, max {du.max():.2e}")
+    assert (d3 > 1e-4).sum() <= 3 * max(n_ulp, 1)
+    assert d3.max() <= 2 * du.max()
+    assert np.median(d3) < 1e-7
     assert np.all(np.abs(v - g["value"]) < 0.1 * err_value)
-    assert np.mean(d3 > 1e-3) < 0.01
