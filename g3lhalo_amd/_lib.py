@@ -71,6 +71,23 @@ class ServerStatus(C.Structure):
                 ("admitted", C.c_uint32), ("rounds", C.c_uint64)]
 
 
+class Cosmology(C.Structure):
+    _fields_ = [("Omega_m", C.c_double), ("Omega_Lambda", C.c_double), ("Omega_b", C.c_double),
+                ("sigma8", C.c_double), ("H0", C.c_double), ("ns", C.c_double)]
+
+
+class PrecomputeIn(C.Structure):
+    _fields_ = [("cosmo", Cosmology), ("zmin", C.c_double), ("zmax", C.c_double),
+                ("mmin", C.c_double), ("mmax", C.c_double), ("kmin", C.c_double),
+                ("kmax", C.c_double), ("nbins", C.c_int32), ("nz_z", _dp), ("nz_n", _dp),
+                ("nz_count", C.c_int32)]
+
+
+class PrecomputeOut(C.Structure):
+    _fields_ = [(n, _dp) for n in ("z", "m", "k", "g", "w", "dwdz", "hmf", "P_lin", "b_h",
+                                   "concentration")]
+
+
 # Every symbol include/g3l.h declares, with its ctypes signature.
 SIGNATURES = {
     "g3l_version": (C.c_char_p, []),
@@ -99,6 +116,7 @@ SIGNATURES = {
     "g3l_server_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_size_t,
                                   C.POINTER(C.c_size_t)]),
     "g3l_server_destroy": (C.c_int, [C.c_void_p]),
+    "g3l_precompute": (C.c_int, [C.POINTER(PrecomputeIn), C.POINTER(PrecomputeOut)]),
 }
 
 _lib = None
@@ -252,6 +270,22 @@ class Context:
         s = BatchStats()
         check("g3l_last_batch_stats", load().g3l_last_batch_stats(self.h, C.byref(s)))
         return s
+
+
+def precompute(cosmo: dict, zmin, zmax, mmin, mmax, kmin, kmax, nbins, nz_z, nz_n) -> dict:
+    """g3l_precompute: the model's tables (z-major 2-D arrays) from a cosmology
+    {Omega_m, Omega_Lambda, Omega_b, sigma8, H0, ns} and source n(z) samples."""
+    z = np.ascontiguousarray(nz_z, dtype=np.float64)
+    n = np.ascontiguousarray(nz_n, dtype=np.float64)
+    cin = PrecomputeIn(Cosmology(*(float(cosmo[k]) for k in ("Omega_m", "Omega_Lambda", "Omega_b",
+                                                              "sigma8", "H0", "ns"))),
+                       zmin, zmax, mmin, mmax, kmin, kmax, int(nbins), _p(z), _p(n), len(z))
+    out = {k: np.zeros(nbins) for k in ("z", "m", "k", "g", "w", "dwdz")}
+    out.update({k: np.zeros((nbins, nbins)) for k in ("hmf", "P_lin", "b_h", "concentration")})
+    cout = PrecomputeOut(*(_p(out[k]) for k in ("z", "m", "k", "g", "w", "dwdz", "hmf", "P_lin",
+                                                 "b_h", "concentration")))
+    check("g3l_precompute", load().g3l_precompute(C.byref(cin), C.byref(cout)))
+    return out
 
 
 class Server:

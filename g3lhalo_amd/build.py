@@ -21,7 +21,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument"]
 
-SOURCES = ["kernels.hip", "engine.cpp", "server.cpp", "round_graph.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "server.cpp", "round_graph.cpp", "precompute.cpp"]
 
 
 def hipcc() -> str:
@@ -32,7 +32,7 @@ def hipcc() -> str:
 
 
 CXX_SOURCES = ["params_hod_cosmo.cpp", "ingest.cpp", "models.cpp"]
-DRIVERS = ["calculateNNMap", "calculateNNMap_twoPop", "calculateNMapMap"]
+DRIVERS = ["calculateNNMap", "calculateNNMap_twoPop", "calculateNMapMap", "doPrecomputations"]
 CXX_LIB = os.path.join(HERE, "libg3lhalo.so")
 BIN = os.path.join(HERE, "bin")
 

@@ -129,16 +129,33 @@ uint32_t slot_cap(uint64_t maxeval, int pts, uint32_t unlimited_cap) {
 
 }  // namespace g3l
 
-namespace {
+namespace g3l {
+
+int ctx_create_bare(g3l_ctx **out) {
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return G3L_ENODEV;
+  g3l_ctx *c = new (std::nothrow) g3l_ctx();
+  if (!c) return G3L_ENOMEM;
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return G3L_EHIP;
+  }
+  bool ok = true;
+  for (auto &e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  for (auto &e : c->kev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  if (!ok) {
+    g3l_ctx_destroy(c);
+    return G3L_EHIP;
+  }
+  *out = c;
+  return G3L_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Batched engine: all jobs refine together; per round one selection launch
 // and one evaluation launch per integrand kind.
-struct EngineJobs {
-  std::vector<JobDesc> desc;
-  std::vector<TermArgs> args;
-  std::vector<double> boxes;  // per job lo[7], hi[7]
-};
 
 int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed, int prec) {
   const uint32_t nj = (uint32_t)E.desc.size();
@@ -318,6 +335,10 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
   }
   return G3L_OK;
 }
+
+}  // namespace g3l
+
+namespace {
 
 // ---------------------------------------------------------------------------
 // Plain engine: the reference's driver (src/hcubature.cpp:904-1021) on the
@@ -577,10 +598,8 @@ int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
   if (!(t->zmax > t->zmin) || !(t->kmax > t->kmin) || !(t->mmax > t->mmin) || t->mmin <= 0 ||
       t->kmin <= 0)
     return G3L_EINVAL;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return G3L_ENODEV;
-  g3l_ctx *c = new (std::nothrow) g3l_ctx();
-  if (!c) return G3L_ENOMEM;
+  g3l_ctx *c = nullptr;
+  if (int rc = ctx_create_bare(&c)) return rc;
   const int nb = t->nbins;
   c->nb = nb;
   c->zmin = t->zmin; c->zmax = t->zmax; c->kmin = t->kmin; c->kmax = t->kmax;
@@ -599,25 +618,6 @@ int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
     double E = c->Om * (1 + z) * (1 + z) * (1 + z) + c->OL;
     c->rho[i] = c->Om * (1 + z) * (1 + z) * (1 + z) / E * c->rho_crit;
   }
-  if (hipGetDevice(&c->device) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    return G3L_EHIP;
-  }
-  for (auto &e : c->ev)
-    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
-  for (auto &e : c->kev)
-    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
-  for (auto e : c->ev)
-    if (!e) {
-      g3l_ctx_destroy(c);
-      return G3L_EHIP;
-    }
-  for (auto e : c->kev)
-    if (!e) {
-      g3l_ctx_destroy(c);
-      return G3L_EHIP;
-    }
   const size_t n1 = nb, n2 = (size_t)nb * nb;
   const size_t total = 6 * n1 + 4 * n2 + n1;
   if (c->tables.ensure(total * sizeof(double))) {

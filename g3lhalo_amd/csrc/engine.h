@@ -157,6 +157,20 @@ namespace g3l {
 
 constexpr int kSetsPerChunk = 64;
 
+// Jobs of one batched-engine run: descriptors, integrand parameters and
+// per-job boxes lo[kMaxDim], hi[kMaxDim].
+struct EngineJobs {
+  std::vector<JobDesc> desc;
+  std::vector<TermArgs> args;
+  std::vector<double> boxes;
+};
+// Refines every job to convergence or its budget on c's device buffers and
+// stream; out receives each job's final state (engine.cpp).
+int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed, int prec);
+// A context with a stream and timing events but no tables, for engine runs
+// whose integrands need none (the table precomputation).
+int ctx_create_bare(g3l_ctx **out);
+
 int kind_of(int stat, int term);
 const double *nbar_ptr(const g3l_ctx *c, int set, int which);
 TermArgs make_args(const g3l_ctx *c, const g3l_request &r);

@@ -73,6 +73,17 @@ struct GatherLaunch {
   uint32_t n;
 };
 
+// Closed-form precomputation tables from the integrals (nb x nb, z-major).
+struct PrecompTablesLaunch {
+  const double *aux;                                // PrecompParams
+  const double *zs, *ms, *ks;                       // grids [nb]
+  const double *growth, *delta_c, *rho_mean, *mstar;  // per redshift [nb]
+  const double *sigma2, *dsigma2;                   // [nb * nb]
+  double *plin, *conc, *hmf, *bh;                   // [nb * nb]
+  int nb;
+};
+hipError_t launch_precomp_tables(const PrecompTablesLaunch &L, hipStream_t s);
+
 // A kernel symbol with its block size and regions per block, for launches
 // recorded as graph nodes.
 struct KernelRef {

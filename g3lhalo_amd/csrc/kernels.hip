@@ -1027,6 +1027,39 @@ hipError_t launch_points(int kind, int prec, const DevTables &T, const TermArgs 
   return hipGetLastError();
 }
 
+// ------------------------------------------------- precomputation tables
+// The closed-form 2-D tables of the precomputation from its integrals, one
+// thread per (z, m or k) entry: linear P(k, z) (src/Powerspectrum.cpp:200-207),
+// c(m, z) = 9/(1+z) (m/m*)^-0.13 (:128-135), the Sheth-Tormen mass function
+// and bias (src/HMF.cpp:7-49, with pi = 3.14159 of src/constants.h:15).
+__global__ void k_precomp_tables(PrecompTablesLaunch L) {
+  const int n = L.nb * L.nb;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int iz = i / L.nb, j = i - iz * L.nb;
+  const PrecompParams &p = *reinterpret_cast<const PrecompParams *>(L.aux);
+  const double z = L.zs[iz], m = L.ms[j];
+  L.plin[i] = eh_spectrum(p, L.ks[j], L.growth[iz]);
+  const double c0 = 9, alpha = 0.13;
+  L.conc[i] = c0 / (1. + z) * pow(m / L.mstar[iz], -alpha);
+  const double A = 0.322, q = 0.707, pp = 0.3, pi = 3.14159;
+  const double sigma2 = L.sigma2[i], dsigma2 = L.dsigma2[i], delta_c = L.delta_c[iz];
+  const double rho_mean = L.rho_mean[iz];
+  const double nuSq = delta_c * delta_c / sigma2;
+  L.hmf[i] = -rho_mean / m / sigma2 * dsigma2 * A * (1 + pow(q * nuSq, -pp)) *
+             sqrt(q * nuSq / 2 / pi) * exp(-0.5 * q * nuSq);
+  L.bh[i] = 1 + 1. / delta_c *
+                    (q * delta_c * delta_c / sigma2 - 1 +
+                     2 * pp / (1 + pow(q * delta_c * delta_c / sigma2, pp)));
+}
+
+hipError_t launch_precomp_tables(const PrecompTablesLaunch &L, hipStream_t s) {
+  const int n = L.nb * L.nb;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_precomp_tables, dim3((n + 255) / 256), dim3(256), 0, s, L);
+  return hipGetLastError();
+}
+
 // ----------------------------------------------------------------- launch
 template <int KV, class P>
 static hipError_t launch_eval_kind(const EvalLaunch &L, hipStream_t s) {

@@ -126,7 +126,96 @@ struct TermArgs {
   int same;                 // NsatNsat same-type branch
   // log-space HOD constants of PMix: log(mmin/M_th), log(mmin/M'), log(mmin)
   double lth1, lmp1, lth2, lmp2, lmmin;
+  // 1-D integrands: kOneDNbar, or a table-precomputation integral whose
+  // cosmology and transfer-function parameters are at aux (PrecompParams)
+  int mode;
+  const double *aux;
 };
+
+// ---------------------------------------------------- table precomputation
+// The 1-D integrands of the reference's table precomputation
+// (src/Cosmology.cpp:84-118, src/Powerspectrum.cpp:71-284), evaluated by the
+// batched GK15 engine like n_bar, with the per-integral parameters in t1..t3.
+enum OneDMode : int {
+  kOneDNbar = 0,       // n_bar(z) (t1 = z)
+  kOneDComoving = 1,   // c / H0 / sqrt(E(z))               -> w(z)
+  kOneDAmplitude = 2,  // W(kR)^2 T(k)^2 k^(ns+2), t1 = R    -> sigma8 amplitude
+  kOneDSigma2 = 3,     // k^2 W(kR)^2 P(k, z), t1 = R, t2 = D(z)/D(0)
+  kOneDDSigma2 = 4,    // k^2 W W'(kR) dR P(k, z), t1 = R, t2 = D(z)/D(0), t3 = dR
+};
+
+// Eisenstein & Hu transfer-function parameters and the cosmology, set on the
+// host (src/Powerspectrum.cpp:15-69, src/Cosmology.cpp:11-72).
+struct PrecompParams {
+  double Om, OL, H0, c, ns, amplitude;
+  double omhh, obhh, k_equality, sound_horizon, k_silk, alpha_c, beta_c, alpha_b, beta_node,
+      beta_b;
+};
+
+// T(k) (src/Powerspectrum.cpp:160-197), in the reference's evaluation order.
+G3L_HD double eh_transfer(const PrecompParams &p, double k) {
+  if (k == 0.0) return 1.0;
+  const double q = k / 13.41 / p.k_equality;
+  const double xx = k * p.sound_horizon;
+  const double T_c_ln_beta = log(2.718282 + 1.8 * p.beta_c * q);
+  const double T_c_ln_nobeta = log(2.718282 + 1.8 * q);
+  const double pq = pow(q, 1.08);
+  const double T_c_C_alpha = 14.2 / p.alpha_c + 386.0 / (1.0 + 69.9 * pq);
+  const double T_c_C_noalpha = 14.2 + 386.0 / (1 + 69.9 * pq);
+  const double T_c_f = 1.0 / (1.0 + pow(xx / 5.4, 4));
+  const double T_c = T_c_f * T_c_ln_beta / (T_c_ln_beta + T_c_C_noalpha * q * q) +
+                     (1. - T_c_f) * T_c_ln_beta / (T_c_ln_beta + T_c_C_alpha * q * q);
+  const double s_tilde = p.sound_horizon * pow(1. + pow(p.beta_node / xx, 3), -1. / 3.);
+  const double xx_tilde = k * s_tilde;
+  const double T_b_T0 = T_c_ln_nobeta / (T_c_ln_nobeta + T_c_C_noalpha * q * q);
+  const double T_b = sin(xx_tilde) / (xx_tilde) *
+                     (T_b_T0 / (1. + pow(xx / 5.2, 2)) +
+                      p.alpha_b / (1. + pow(p.beta_b / xx, 3)) * exp(-pow(k / p.k_silk, 1.4)));
+  const double f_baryon = p.obhh / p.omhh;
+  return f_baryon * T_b + (1. - f_baryon) * T_c;
+}
+
+// Top-hat window and its derivative (src/Powerspectrum.cpp:97-107).
+G3L_HD double tophat(double k, double R) {
+  const double x = k * R;
+  return 3 / x / x / x * (sin(x) - x * cos(x));
+}
+G3L_HD double d_tophat(double k, double R) {
+  const double x = k * R;
+  return ((x * x - 3.) * sin(x) + 3. * x * cos(x)) * 3 * k / x / x / x / x;
+}
+
+// Linear P(k, z) with growth = D(z)/D(0) (src/Powerspectrum.cpp:200-207).
+G3L_HD double eh_spectrum(const PrecompParams &p, double k, double growth) {
+  const double trans = eh_transfer(p, k);
+  return p.amplitude * trans * trans * growth * growth * pow(k, p.ns);
+}
+
+// One precomputation integrand value at x (src/Cosmology.cpp:104-118,
+// src/Powerspectrum.cpp:71-95, 229-245, 265-284).
+G3L_HD double precomp_kernel(const TermArgs &a, double x) {
+  const PrecompParams &p = *reinterpret_cast<const PrecompParams *>(a.aux);
+  switch (a.mode) {
+    case kOneDComoving: {
+      const double zp = 1 + x;
+      return p.c / p.H0 / sqrt(p.Om * zp * zp * zp + p.OL);
+    }
+    case kOneDAmplitude: {
+      const double W = tophat(x, a.t1);
+      const double T = eh_transfer(p, x);
+      return W * W * T * T * pow(x, p.ns + 2);
+    }
+    case kOneDSigma2: {
+      const double W = tophat(x, a.t1);
+      return x * x * W * W * eh_spectrum(p, x, a.t2);
+    }
+    default: {
+      const double W = tophat(x, a.t1);
+      const double Wprime = d_tophat(x, a.t1);
+      return x * x * W * Wprime * a.t3 * eh_spectrum(p, x, a.t2);
+    }
+  }
+}
 
 // ---------------------------------------------------------------- indices
 // Nearest-index look-ups: round() in the integrands, int() truncation in
@@ -977,10 +1066,12 @@ G3L_HD double integrand_src(const DevTables &T, const TermArgs &a, const S &s) {
   }
 }
 
-// n_bar integrand, x = log10 m, theta slot 1 = z (src/generalFunctions.cu:233-244).
+// n_bar integrand, x = log10 m, theta slot 1 = z (src/generalFunctions.cu:233-244);
+// the table-precomputation integrands share its 1-D kind (mode != kOneDNbar).
 template <class P>
 G3L_HD double nbar_kernel(const DevTables &T, const TermArgs &a,
                                               const double *x) {
+  if (a.mode != kOneDNbar) return precomp_kernel(a, x[0]);
   double z = a.t1;
   int zr = zix_round(T, z);
   Mass M = make_mass<P>(T, x[0], zix_trunc(T, z));

@@ -174,6 +174,36 @@ int g3l_set_variant(g3l_ctx *ctx, int32_t kind, int32_t precision, int32_t varia
 typedef enum g3l_launch_mode { G3L_LAUNCH_STREAM = 0, G3L_LAUNCH_GRAPH = 1 } g3l_launch_mode;
 int g3l_set_launch_mode(g3l_ctx *ctx, int32_t mode);
 
+/* ---------------------------------------------------------------------
+ * Table precomputation.  Replaces doPrecomputations.x
+ * (src/doPrecomputations.cpp:7-135 with src/Cosmology.cpp,
+ * src/Powerspectrum.cpp, src/HMF.cpp): the nine input tables of the model
+ * from a cosmology and a source redshift distribution, on the grids
+ * z_i = zmin + i (zmax - zmin)/N, m_j = exp(log mmin + j log(mmax/mmin)/N),
+ * k_j = exp(log kmin + j log(kmax/kmin)/N).  Every adaptive integral (the
+ * comoving distances, the sigma8 normalisation, sigma^2(m, z) and
+ * d sigma^2/dm on the model grid and on the 256-mass m* grid) refines on the
+ * device's batched Gauss-Kronrod engine at the reference's tolerance (1e-4
+ * relative); the closed-form tables are computed on the device from them.
+ * Outputs are caller-allocated host arrays; returns a g3l_status.  Needs no
+ * context. */
+typedef struct g3l_cosmology {
+  double Omega_m, Omega_Lambda, Omega_b, sigma8, H0, ns;  /* H0 in km/s/Mpc */
+} g3l_cosmology;
+typedef struct g3l_precompute_in {
+  g3l_cosmology cosmo;
+  double zmin, zmax, mmin, mmax, kmin, kmax;  /* [Msun], [1/Mpc] */
+  int32_t nbins;
+  const double *nz_z, *nz_n;  /* source n(z): nz_count samples (src/Function.cpp:6-37) */
+  int32_t nz_count;
+} g3l_precompute_in;
+typedef struct g3l_precompute_out {
+  double *z, *m, *k;                            /* grids [nbins] */
+  double *g, *w, *dwdz;                         /* [nbins] */
+  double *hmf, *P_lin, *b_h, *concentration;    /* [nbins * nbins], z-major */
+} g3l_precompute_out;
+int g3l_precompute(const g3l_precompute_in *in, g3l_precompute_out *out);
+
 /* Version string of this build. */
 const char *g3l_version(void);
 

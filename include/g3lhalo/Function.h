@@ -20,8 +20,11 @@ class Function {
   int readLog(const int &Nbins, const double &min, const double &max,
               std::vector<double> &values);
 
- private:
+  // the samples as read, public as in the reference (src/Function.h), which
+  // doPrecomputations passes on directly (src/doPrecomputations.cpp:55-62)
   std::vector<double> x_values_, y_values_;
+
+ private:
   double default_value_ = 0;
 };
 }  // namespace g3lhalo
