@@ -38,7 +38,7 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
 # WRITE_SIZE runs of this workload scaled by a copy kernel of known size
 # (profiles/k_eval_nnmap550_counters.md): 464.1 MB read + 43.1 MB written per
 # k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
-TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 507252858}
+TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 507255602}
 
 
 # Launch variant per kind when the autotuner does not run (the library's
@@ -219,6 +219,53 @@ def hod_sweep(ctx, n_hods: int = 256, pool_slots: int = 1 << 26, precision=_lib.
                         "inside the timed region; one server, bounded region pool"}
 
 
+def precompute_line(nbins: int = 128, cpu_nbins: int = 32):
+    """The table precomputation (g3l_precompute) on the nnmap550 grids: the
+    fiducial cosmology, z in [0.001, 2], m in [1e10, 1e17], k in [1e-4, 1e4],
+    a source n(z) ~ z^2 exp(-(z/0.5)^1.5) on 100 points; beside it the
+    reference's own doPrecomputations.x (built from its sources into
+    oracle/_ref) on the host cores at a smaller grid, when present."""
+    import subprocess
+    import tempfile
+    c = T.Cosmo()
+    cosmo = {"Omega_m": c.Omega_m, "Omega_Lambda": c.Omega_Lambda, "Omega_b": c.Omega_b,
+             "sigma8": c.sigma8, "H0": c.H0, "ns": c.ns}
+    z = np.linspace(0.01, 3.0, 100)
+    n = z ** 2 * np.exp(-(z / 0.5) ** 1.5)
+    n /= n.sum() * (z[-1] - z[0]) / len(z)
+    grid = (0.001, 2.0, 1e10, 1e17, 1e-4, 1e4)
+
+    def integrals(nb):
+        return 1 + nb + len(z) + 2 * nb * nb + 256 * nb
+
+    _lib.precompute(cosmo, *grid, 8, z, n)   # warm-up
+    t0 = time.perf_counter()
+    _lib.precompute(cosmo, *grid, nbins, z, n)
+    dt = time.perf_counter() - t0
+    line = {"nbins": nbins, "seconds": round(dt, 3), "adaptive_integrals": integrals(nbins),
+            "integrals_per_s": round(integrals(nbins) / dt, 1),
+            "note": "all seven tables; the integrals refine on the device's batched GK15 engine"}
+    ref = os.path.join(ROOT, "oracle", "_ref", "doPrecomputations.x")
+    if os.path.exists(ref):
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        with tempfile.TemporaryDirectory() as d:
+            with open(os.path.join(d, "cosmo.param"), "w") as f:
+                f.write("".join(f"{k} {v!r}\n" for k, v in cosmo.items()))
+            with open(os.path.join(d, "nz.dat"), "w") as f:
+                f.write("".join(f"{a:.17g} {b:.17g}\n" for a, b in zip(z, n)))
+            args = [ref, "cosmo.param", "nz.dat", *[repr(v) for v in grid], str(cpu_nbins),
+                    "g.dat", "w.dat", "dwdz.dat", "hmf.dat", "pk.dat", "bh.dat", "conc.dat"]
+            t0 = time.perf_counter()
+            subprocess.run(args, cwd=d, check=True, capture_output=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+            cdt = time.perf_counter() - t0
+        line["cpu_reference"] = {"nbins": cpu_nbins, "seconds": round(cdt, 3),
+                                 "adaptive_integrals": integrals(cpu_nbins),
+                                 "integrals_per_s": round(integrals(cpu_nbins) / cdt, 1),
+                                 "cores": threads, "kind": "reference"}
+    return line
+
+
 def copy_probe(mib: int) -> int:
     """One elementwise float32 kernel that reads mib MiB and writes mib MiB: the
     known-size reference against which the HBM traffic counters are scaled."""
@@ -252,6 +299,8 @@ def main(argv=None):
                     help="HOD vectors in the batched fit sweep (0 skips it)")
     ap.add_argument("--no-nmapmap", action="store_true",
                     help="skip the NMapMap 550-triplet line")
+    ap.add_argument("--no-precompute", action="store_true",
+                    help="skip the table-precomputation line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run an elementwise kernel that reads and writes this many MiB "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -335,6 +384,7 @@ def main(argv=None):
     if mixed is not None:
         mdev = np.array([abs(m.value - r.value) / max(abs(r.value), 1e-300)
                          for m, r in zip(mixed["results"], f64["results"])])
+        terms = np.array([r.term for r in reqs])
         mk = int(np.argmax(mixed["kind_ms"][:7]))
         mflops = float(mixed["kind_evals"][mk]) * flops.FLOPS_PER_EVAL[mk]
         mach = mflops / (mixed["kind_ms"][mk] * 1e-3) / 1e12 if mixed["kind_ms"][mk] > 0 else 0.0
@@ -345,6 +395,11 @@ def main(argv=None):
                 "p50": float(np.quantile(mdev, 0.5)), "p90": float(np.quantile(mdev, 0.9)),
                 "p99": float(np.quantile(mdev, 0.99)), "max": float(mdev.max()),
                 "frac_above_1e-4": float(np.mean(mdev > 1e-4))},
+            "rel_diff_vs_fp64_per_term": {
+                f"{t}h": {"p50": float(np.quantile(mdev[terms == t], 0.5)),
+                          "max": float(mdev[terms == t].max()),
+                          "frac_above_1e-4": float(np.mean(mdev[terms == t] > 1e-4))}
+                for t in (1, 2, 3)},
             "roofline": {"kernel": f"k_eval<{kernel_kv(tuned, 'mixed', mk)},PMix> "
                                    f"({KIND_NAMES[mk]})",
                          "achieved": round(mach, 3), "peak": round(flops.mixed_peak_tflops(mk), 2),
@@ -413,6 +468,8 @@ def main(argv=None):
                               "evals": int(sum(int(r.neval) for r in nres)), **ndesc}
     if a.sweep_hods > 0:
         line["hod_sweep"] = hod_sweep(ctx, a.sweep_hods)
+    if not a.no_precompute:
+        line["precompute"] = precompute_line()
     if a.serve_requests > 0:
         line["served"] = serve(ctx, tables, a.serve_requests, a.serve_rate)
         line["latency_ms"]["served_p50"] = line["served"]["p50_ms"]
