@@ -56,3 +56,33 @@ def test_struct_layouts():
     assert _lib.Request.theta.offset == 24
     assert C.sizeof(_lib.Result) == 8 * 4 + 4 + 4
     assert C.sizeof(_lib.Hod) == 48
+
+
+def test_precompute_rejects_bad_arguments():
+    """g3l_precompute validates its inputs before touching a device."""
+    import numpy as np
+    L = _lib.load()
+    n = 4
+    z = np.linspace(0.01, 3.0, 10)
+    nz = np.ones(10)
+    bufs = {k: np.zeros(n) for k in ("z", "m", "k", "g", "w", "dwdz")}
+    bufs.update({k: np.zeros(n * n) for k in ("hmf", "P_lin", "b_h", "concentration")})
+    out = _lib.PrecomputeOut(*(_lib._p(bufs[k]) for k in ("z", "m", "k", "g", "w", "dwdz", "hmf",
+                                                           "P_lin", "b_h", "concentration")))
+
+    def make(**kw):
+        cfg = dict(zmin=0.001, zmax=2.0, mmin=1e10, mmax=1e17, kmin=1e-4, kmax=1e4, nbins=n,
+                   Om=0.3, OL=0.7, Ob=0.045)
+        cfg.update(kw)
+        cos = _lib.Cosmology(cfg["Om"], cfg["OL"], cfg["Ob"], 0.8, 70.0, 0.96)
+        return _lib.PrecomputeIn(cos, cfg["zmin"], cfg["zmax"], cfg["mmin"], cfg["mmax"],
+                                 cfg["kmin"], cfg["kmax"], cfg["nbins"], _lib._p(z), _lib._p(nz),
+                                 len(z))
+
+    assert L.g3l_precompute(None, C.byref(out)) == _lib.G3L_EINVAL
+    assert L.g3l_precompute(C.byref(make()), None) == _lib.G3L_EINVAL
+    for bad in (dict(nbins=0), dict(zmax=0.0), dict(mmin=0.0), dict(mmax=1e9),
+                dict(kmin=-1.0), dict(kmax=1e-5), dict(Ob=0.3), dict(Om=0.0)):
+        assert L.g3l_precompute(C.byref(make(**bad)), C.byref(out)) == _lib.G3L_EINVAL, bad
+    partial = _lib.PrecomputeOut()
+    assert L.g3l_precompute(C.byref(make()), C.byref(partial)) == _lib.G3L_EINVAL
