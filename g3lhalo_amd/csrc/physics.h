@@ -621,9 +621,12 @@ struct PMix {
   //   a1 >= 4:       u F = g(a1) + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c)
   //   a2 <  4:       u = 1 + [(cos a1 - 1) ln(1+c) + (a1 c - sin a1 c)/a2
   //                        + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si(a2) - Si(a1))] / F
-  //   a1 < 4 <= a2:  the FP64 form (the O(1) Si/Ci terms cancel there).
+  //   a1 < 4 <= a2:  u F = pi/2 sin a1 - sin a1 Si(a1) - cos a1 Ci(a1)
+  //                        + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c),
+  //                  with Ci = gamma + ln a1 + Cr; its O(1) terms sum to
+  //                  u F ~ 0.2-2 there, so FP32 keeps ~6 digits.
   G3L_HD static double u(const Nfw &n, double k) {
-    double a1d = k * n.r200 / n.c;
+    double a1d = fdiv(k * n.r200, n.c);
     double a2d = a1d * (1 + n.c);
     double xd = a1d * n.c;
     float sx, cxx;
@@ -633,7 +636,7 @@ struct PMix {
       mix::aux32((float)a1d, f1, g1, fm1);
       mix::aux32((float)a2d, f2, g2, fm2);
       float uF = g1 + sx * fm2 - g2 * cxx;
-      return (double)uF / n.F;
+      return (double)(uF / (float)n.F);
     }
     float s1, c1;
     mix::sincos_red(a1d, s1, c1);
@@ -646,10 +649,13 @@ struct PMix {
       float h = sinf(0.5f * a1);
       float rest = -2.0f * h * h * (float)n.lc + mix::x_minus_sin((float)xd, sx) / a2 +
                    c1 * (cr2 - cr1) + s1 * (si2 - si1);
-      return 1.0 + (double)rest / n.F;
+      return 1.0 + (double)(rest / (float)n.F);
     }
-    // transition a1 < 4 <= a2: O(1) terms cancel down to u F, in FP64
-    return P64::u(n, k);
+    float f2, g2, fm2;
+    mix::aux32(a2, f2, g2, fm2);
+    const float ci1 = 0.5772156649f + logf(a1) + cr1;
+    const float uF = 1.5707963267948966f * s1 - s1 * si1 - c1 * ci1 + sx * fm2 - g2 * cxx;
+    return (double)(uF / (float)n.F);
   }
 
   G3L_HD static double ap(double eta) {
