@@ -177,6 +177,21 @@ def serve(ctx, tables, n: int, rate: float, precision=_lib.FP64, seed: int = 4):
                     "grow"}
 
 
+def vs_fixture(values, name: str, tables):
+    """Per-term relative error of 550-triplet results (1h, 2h, 3h interleaved)
+    against the oracle's committed values for the same workload, or None if
+    the fixture is absent or was made from other tables."""
+    path = os.path.join(ROOT, "tests", "golden", f"{name}_oracle.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+    if str(g["tables_checksum"]) != T.checksum(tables) or len(g["value"]) != len(values):
+        return None
+    d = np.abs(np.asarray(values) - g["value"]) / np.abs(g["value"])
+    return {f"{t}h": {"p50": float(np.median(d[t - 1::3])), "max": float(d[t - 1::3].max()),
+                      "frac_above_1e-4": float(np.mean(d[t - 1::3] > 1e-4))} for t in (1, 2, 3)}
+
+
 def hod_sweep(ctx, n_hods: int = 256, pool_slots: int = 1 << 26, precision=_lib.FP64):
     """The batched fit sweep: n_hods HOD vectors from the uniform prior x the
     55 triplets (a_i, a_j, a_j) x NNMap 1h/2h/3h, every integral at tol 1e-4
@@ -408,6 +423,12 @@ def main(argv=None):
             "note": "FP32 transcendental bulk; not the headline: these integrals end at the "
                     "evaluation cap unconverged, and FP32 noise re-orders near-tied regions"}
 
+    if a.workload == "nnmap550":
+        fx = {"fp64": vs_fixture([r.value for r in f64["results"]], "nnmap550", tables)}
+        if mixed is not None:
+            fx["mixed"] = vs_fixture([r.value for r in mixed["results"]], "nnmap550", tables)
+        if mixed_info is not None and fx.get("mixed") is not None:
+            mixed_info["rel_err_vs_oracle_per_term"] = fx["mixed"]
     line = {
         "metric": "NNMap integrals/sec at 1e-4 rel tol (batched vs plain, 1 GPU); p50/p99 latency",
         "value": round(value, 3),
@@ -444,6 +465,7 @@ def main(argv=None):
                      "per_kernel": per_kind},
         "autotune": {p: {KIND_NAMES[k]: r["group"][r["variant"]] for k, r in t.items()}
                      for p, t in tuned.items()},
+        "rel_err_vs_oracle_per_term": (fx["fp64"] if a.workload == "nnmap550" else None),
         "evals_per_step": int(sum(int(r.neval) for r in results)),
         "failed_integrals": len(bad),
     }
@@ -465,7 +487,18 @@ def main(argv=None):
         dt = time.perf_counter() - t0
         line["nmapmap550"] = {"value": round(len(nreqs) / dt, 3), "unit": "integrals/s",
                               "ms_per_step": round(dt * 1e3, 3), "steps": 1, "warmup": 1,
-                              "evals": int(sum(int(r.neval) for r in nres)), **ndesc}
+                              "evals": int(sum(int(r.neval) for r in nres)), **ndesc,
+                              "rel_err_vs_oracle_per_term":
+                                  vs_fixture([r.value for r in nres], "nmapmap550", tables)}
+        if not a.fp64_only:
+            ctx.eval_batch(nreqs, precision=_lib.MIXED)
+            t0 = time.perf_counter()
+            mres = ctx.eval_batch(nreqs, precision=_lib.MIXED)
+            mdt = time.perf_counter() - t0
+            line["nmapmap550"]["mixed"] = {
+                "value": round(len(nreqs) / mdt, 3), "unit": "integrals/s",
+                "rel_err_vs_oracle_per_term": vs_fixture([r.value for r in mres], "nmapmap550",
+                                                         tables)}
     if a.sweep_hods > 0:
         line["hod_sweep"] = hod_sweep(ctx, a.sweep_hods)
     if not a.no_precompute:
