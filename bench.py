@@ -78,6 +78,17 @@ def workload(name: str, hodset: int = 0):
     return reqs, desc
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (the CPU baselines run on it)."""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(tables, reqs, seconds_budget: float):
     """Oracle (FP64 C++/OpenMP port of the reference CPU build) on a sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -100,6 +111,7 @@ def cpu_baseline(tables, reqs, seconds_budget: float):
             break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": done / t_used, "unit": "integrals/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"{done} of 96 evenly spaced integrals of the workload (first index "
                       f"{min(vals)}, last {max(vals)}), full maxEval, until {seconds_budget:g} s",
             "seconds": round(t_used, 2)}, vals
@@ -277,7 +289,7 @@ def precompute_line(nbins: int = 128, cpu_nbins: int = 32):
         line["cpu_reference"] = {"nbins": cpu_nbins, "seconds": round(cdt, 3),
                                  "adaptive_integrals": integrals(cpu_nbins),
                                  "integrals_per_s": round(integrals(cpu_nbins) / cdt, 1),
-                                 "cores": threads, "kind": "reference"}
+                                 "cores": threads, "kind": "reference", "cpu": cpu_model()}
     return line
 
 
