@@ -1,9 +1,11 @@
 """Writes tests/golden/precompute_<name>/: the inputs and the output files of
 the reference's table precomputation (doPrecomputations.x, built from the
 reference's own sources into oracle/_ref by `make -C oracle ref`) for two
-small configurations -- the fiducial flat LCDM cosmology at 16 bins and the
+small configurations -- the fiducial flat LCDM cosmology at 16 bins, the
 reference's example cosmology (scripts_examples/cosmo.param values) at 8
-bins.  The device precomputation (g3l_precompute) is checked against these
+bins, and the fiducial cosmology on the grid that tests/test_gpu_pipeline.py
+hands to the drop-in drivers (redshifts 0.01-2.01, whose last abscissa prints
+above its exact value, so src/Function.cpp:55-57 reads it back).  The device precomputation (g3l_precompute) is checked against these
 files.  A few seconds on 4 cores.
 
 Usage: python tests/golden/make_precompute_golden.py
@@ -30,6 +32,10 @@ CONFIGS = {
                            "sigma8": 0.9, "H0": 73, "ns": 1.0},
                  "zmin": 0.1, "zmax": 2.0, "mmin": 1e10, "mmax": 1e17, "kmin": 0.01,
                  "kmax": 1000.0, "nbins": 8},
+    "pipeline16": {"cosmo": {"Omega_m": 0.3, "Omega_Lambda": 0.7, "Omega_b": 0.045,
+                             "sigma8": 0.8, "H0": 70, "ns": 0.96},
+                   "zmin": 0.01, "zmax": 2.01, "mmin": 1e10, "mmax": 1e17, "kmin": 1e-4,
+                   "kmax": 1e4, "nbins": 16},
 }
 
 

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(ROOT, "tests", "golden")
 BIN = os.path.join(ROOT, "g3lhalo_amd", "bin", "doPrecomputations.x")
 OUTPUTS = ("g.dat", "w.dat", "dwdz.dat", "hmf.dat", "pk.dat", "bh.dat", "conc.dat")
-CONFIGS = ("fiducial16", "example8")
+CONFIGS = ("fiducial16", "example8", "pipeline16")
 # table -> file; every value within the reference's six-digit rounding
 TABLES = {"g": "g.dat", "w": "w.dat", "dwdz": "dwdz.dat", "P_lin": "pk.dat",
           "concentration": "conc.dat", "hmf": "hmf.dat", "b_h": "bh.dat"}

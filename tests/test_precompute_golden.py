@@ -16,7 +16,7 @@ from conftest import ROOT
 GOLD = os.path.join(ROOT, "tests", "golden")
 REF = os.path.join(ROOT, "oracle", "_ref", "doPrecomputations.x")
 OUTPUTS = ("g.dat", "w.dat", "dwdz.dat", "hmf.dat", "pk.dat", "bh.dat", "conc.dat")
-CONFIGS = ("fiducial16", "example8")
+CONFIGS = ("fiducial16", "example8", "pipeline16")
 
 
 def load(name):
