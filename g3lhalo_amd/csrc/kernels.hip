@@ -127,8 +127,10 @@ __device__ __forceinline__ void reduce_regions(const EvalLaunch &L, const Stage<
 }
 
 // Direct evaluation (n_bar): every point evaluates its integrand in full.
+// Held to 64 VGPRs (8 waves per SIMD, two 1024-thread blocks per CU) as
+// k_eval is: the 256-HOD n_bar sweep runs 148 ms instead of 158 ms at 88.
 template <int KIND, class P>
-__global__ __launch_bounds__(1024) void k_eval_direct(EvalLaunch L) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_eval_direct(EvalLaunch L) {
   constexpr int D = kind_dim(kind_of_kv(KIND));
   constexpr int NP = KGroup<KIND>::P;
   constexpr int G = KGroup<KIND>::G;
@@ -582,8 +584,12 @@ __device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<kind_
   }
 }
 
+// At most 64 VGPRs (8 waves per SIMD): the FP64 5-D and 7-D kinds otherwise
+// compile to 70-80 and run 6-7 waves per SIMD.  The few registers this
+// spills cost less than the lost occupancy: NNMap 1h 4.12 -> 3.69 ms and
+// 3h 4.26 -> 3.93 ms per launch on the 550-triplet workload, 2h unchanged.
 template <int KIND, class P>
-__global__ __launch_bounds__(1024) void k_eval(EvalLaunch L) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_eval(EvalLaunch L) {
   using S = Spec<kind_of_kv(KIND)>;
   constexpr int D = S::D;
   constexpr int NP = KGroup<KIND>::P;
