@@ -360,18 +360,27 @@ def main(argv=None):
         kind_launches = np.zeros(8, dtype=np.int64)
         kind_evals = np.zeros(8, dtype=np.int64)
         res = None
+        parts = []
         t_start = time.perf_counter()
         for _ in range(a.steps):
             t0 = time.perf_counter()
             res = ctx.eval_batch(reqs, precision=precision)
             step_ms.append((time.perf_counter() - t0) * 1e3)
             st = ctx.last_stats()
+            parts.append((st.eval_ms, st.select_ms, st.wall_ms, st.rounds))
             kind_ms += np.array(st.kind_ms[:])
             kind_launches += np.array(st.kind_launches[:], dtype=np.int64)
             kind_evals += np.array(st.kind_evals[:], dtype=np.int64)
         elapsed = time.perf_counter() - t_start
+        ev, sel, wall, rounds = (float(np.mean([p[i] for p in parts])) for i in range(4))
+        # where one step's time goes: evaluation launches, selection kernels
+        # (both from event nodes of the round graphs) and the rest (host
+        # turnaround between rounds, graph launches, the result copy)
+        breakdown = {"eval": round(ev, 3), "select": round(sel, 3),
+                     "rest": round(wall - ev - sel, 3), "rounds": int(rounds)}
         return {"value": len(reqs) * a.steps / elapsed, "step_ms": step_ms, "results": res,
-                "kind_ms": kind_ms, "kind_launches": kind_launches, "kind_evals": kind_evals}
+                "kind_ms": kind_ms, "kind_launches": kind_launches, "kind_evals": kind_evals,
+                "breakdown": breakdown}
 
     f64 = run_form(_lib.FP64)
     mixed = run_form(_lib.MIXED) if not a.fp64_only else None
@@ -462,6 +471,7 @@ def main(argv=None):
                   "speedup_mixed_over_plain": (round(mixed["value"] / plain_rate, 2)
                                                if mixed is not None else None),
                   "max_rel_diff_batched_vs_plain": float(dev)},
+        "step_breakdown_ms": head["breakdown"],
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
                        "p99": round(float(np.percentile(step_ms, 99)), 3),
                        "note": "batched form: every integral of a step completes with the step"},

@@ -704,20 +704,47 @@ __global__ void k_zero(uint32_t *p, int n) {
 }
 
 // ---------------------------------------------------------------- select
-// Block-wide exclusive scan of a 0/1 predicate over 256 threads (4 waves of
-// 64 lanes); returns the prefix and writes the block total.
-__device__ __forceinline__ uint32_t block_scan_bit(bool pred, uint32_t *s_wave, uint32_t &total) {
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  uint64_t mask = __ballot(pred);
-  uint32_t before = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-  if (lane == 0) s_wave[wave] = (uint32_t)__popcll(mask);
+// One 256-thread block (4 waves of 64 lanes) per integral.  The passes over
+// the integral's regions are latency-bound (one block per integral, a few
+// hundred iterations per thread for the largest), so each iteration issues
+// four independent loads, and the scans use wave shuffles rather than
+// thread-serial loops.
+constexpr int kSelThreads = 256;
+
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, unsigned d) {
+  const uint32_t lo = __shfl_up((uint32_t)v, d, 64), hi = __shfl_up((uint32_t)(v >> 32), d, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive prefix sums over the block in thread order of (a, b); s_wa, s_wb
+// hold the wave totals (4 each).  Ends with a barrier, so the buffers can be
+// reused at once; ta, tb receive the block totals.
+__device__ __forceinline__ void block_scan2(uint64_t &a, uint64_t &b, uint64_t *s_wa,
+                                            uint64_t *s_wb, uint64_t &ta, uint64_t &tb) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (unsigned d = 1; d < 64; d <<= 1) {
+    const uint64_t xa = shfl_up_u64(a, d), xb = shfl_up_u64(b, d);
+    if (lane >= (int)d) {
+      a += xa;
+      b += xb;
+    }
+  }
+  if (lane == 63) {
+    s_wa[wave] = a;
+    s_wb[wave] = b;
+  }
   __syncthreads();
-  uint32_t off = 0;
-  for (int w = 0; w < wave; ++w) off += s_wave[w];
-  total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+  uint64_t oa = 0, ob = 0;
+  for (int w = 0; w < wave; ++w) {
+    oa += s_wa[w];
+    ob += s_wb[w];
+  }
+  ta = s_wa[0] + s_wa[1] + s_wa[2] + s_wa[3];
+  tb = s_wb[0] + s_wb[1] + s_wb[2] + s_wb[3];
+  a += oa;
+  b += ob;
   __syncthreads();
-  return off + before;
 }
 
 __device__ __forceinline__ uint64_t err_key(double e) {
@@ -726,19 +753,23 @@ __device__ __forceinline__ uint64_t err_key(double e) {
   return (uint64_t)__double_as_longlong(k);
 }
 
-__global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
+__device__ __forceinline__ unsigned long long fixed_err(uint64_t k, int shift) {
+  return (unsigned long long)ldexp(__longlong_as_double((long long)k), shift);
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
   const uint32_t J = L.active ? L.active[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
   __shared__ JobState s_st;
-  __shared__ double s_red[256];
-  __shared__ double s_red2[256];
-  __shared__ uint64_t s_redk[256];
-  __shared__ uint32_t s_cnt[256];
-  __shared__ unsigned long long s_sum[256];
-  __shared__ uint32_t s_wave[4];
+  __shared__ double s_red[kSelThreads];
+  __shared__ double s_red2[kSelThreads];
+  __shared__ uint64_t s_redk[kSelThreads];
+  __shared__ uint32_t s_cnt[kSelThreads];
+  __shared__ unsigned long long s_sum[kSelThreads];
+  __shared__ uint64_t s_wa[4], s_wb[4];
   __shared__ int s_flag;
-  __shared__ uint64_t s_prefix, s_mask, s_cumsum;
-  __shared__ uint32_t s_cumcnt, s_lastcnt, s_r, s_K, s_listbase, s_tiecarry, s_selcarry;
+  __shared__ uint64_t s_prefix, s_mask, s_cumsum, s_need;
+  __shared__ uint32_t s_cumcnt, s_lastcnt, s_r, s_K, s_listbase;
 
   if (tid == 0) s_st = L.state[J];
   __syncthreads();
@@ -749,27 +780,38 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
   const double *val = L.geo + kFieldVal * S + d.base;
   const double *err = L.geo + kFieldErr * S + d.base;
 
-  // 1. totals: value with Neumaier compensation, error plain, both in a
-  //    fixed order so the result does not depend on scheduling.
+  // 1. totals: value with Neumaier compensation, error plain, each thread
+  //    over i = tid, tid + 256, ... in that order, then a fixed tree: the
+  //    result does not depend on scheduling.  Loads four at a time; absent
+  //    entries enter as zeros, which leave every sum unchanged.
   double vs = 0, vc = 0, es = 0;
   uint64_t kmax = 0;
   bool bad = false;
-  for (uint32_t i = tid; i < n; i += 256) {
-    double v = val[i], e = err[i];
-    bad |= !isfinite(v) || !isfinite(e);
-    double t = vs + v;
-    vc += fabs(vs) >= fabs(v) ? (vs - t) + v : (v - t) + vs;
-    vs = t;
-    es += e;
-    uint64_t k = err_key(e);
-    kmax = k > kmax ? k : kmax;
+  for (uint32_t i0 = tid; i0 < n; i0 += 4 * kSelThreads) {
+    double v[4], e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t i = i0 + u * kSelThreads;
+      v[u] = i < n ? val[i] : 0.0;
+      e[u] = i < n ? err[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      bad |= !isfinite(v[u]) || !isfinite(e[u]);
+      const double t = vs + v[u];
+      vc += fabs(vs) >= fabs(v[u]) ? (vs - t) + v[u] : (v[u] - t) + vs;
+      vs = t;
+      es += e[u];
+      const uint64_t k = err_key(e[u]);
+      kmax = k > kmax ? k : kmax;
+    }
   }
   s_red[tid] = vs;
   s_red2[tid] = vc;
   s_redk[tid] = kmax;
   s_cnt[tid] = bad ? 1u : 0u;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = kSelThreads / 2; w > 0; w >>= 1) {
     if (tid < w) {
       double a = s_red[tid], b = s_red[tid + w];
       double t = a + b;
@@ -787,7 +829,7 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
   __syncthreads();
   s_red[tid] = es;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = kSelThreads / 2; w > 0; w >>= 1) {
     if (tid < w) s_red[tid] += s_red[tid + w];
     __syncthreads();
   }
@@ -822,6 +864,10 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
 
   // 3. batch size: pops continue until the remaining error meets the
   //    tolerance or the evaluation budget is spent (src/hcubature.cpp:971-985).
+  //    The pops take the regions in decreasing error; the batch is found by
+  //    a radix select over the 64-bit error keys, 8 bits per level from the
+  //    top, in fixed-point error units (maxkey maps below 2^47, so a sum over
+  //    2^16 regions cannot overflow; integer sums are exact and order-free).
   const uint64_t twoP = 2ull * (uint64_t)d.pts;
   uint64_t kcap = n;
   if (d.maxeval != 0) {
@@ -829,74 +875,69 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
     uint64_t kc = (room + twoP - 1) / twoP;
     kcap = kc < kcap ? kc : kcap;
   }
-  // fixed-point error units: maxkey maps below 2^47, so a sum over 2^16
-  // regions cannot overflow; integer sums are exact and order-free.
-  int ex = (int)((maxkey >> 52) & 0x7ff) - 1023;
-  int shift = 46 - ex;
-  uint64_t Efx = 0;
-  {
-    unsigned long long part = 0;
-    for (uint32_t i = tid; i < n; i += 256) {
-      double k = __longlong_as_double((long long)err_key(err[i]));
-      part += (unsigned long long)ldexp(k, shift);
-    }
-    s_sum[tid] = part;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (tid < w) s_sum[tid] += s_sum[tid + w];
-      __syncthreads();
-    }
-    Efx = s_sum[0];
-    __syncthreads();
-  }
-  double thr = ldexp(fabs(V) * d.reltol, shift);
-  uint64_t need;
-  if (thr >= (double)Efx) need = 1;  // at least one region is popped
-  else need = Efx - (uint64_t)floor(thr);
-  if (need == 0) need = 1;
-
-  // radix select over the 64-bit keys, 8 bits per level, from the top
+  const int ex = (int)((maxkey >> 52) & 0x7ff) - 1023;
+  const int shift = 46 - ex;
+  const double thr = ldexp(fabs(V) * d.reltol, shift);
   if (tid == 0) {
     s_prefix = 0;
     s_mask = 0;
     s_cumsum = 0;
     s_cumcnt = 0;
   }
-  __syncthreads();
   for (int level = 0; level < 8; ++level) {
     const int sh = 56 - 8 * level;
     s_cnt[tid] = 0;
     s_sum[tid] = 0;
     __syncthreads();
     const uint64_t prefix = s_prefix, mask = s_mask;
-    for (uint32_t i = tid; i < n; i += 256) {
-      uint64_t k = err_key(err[i]);
-      if ((k & mask) == prefix) {
-        int dgt = (int)((k >> sh) & 0xff);
-        atomicAdd(&s_cnt[dgt], 1u);
-        atomicAdd(&s_sum[dgt],
-                  (unsigned long long)ldexp(__longlong_as_double((long long)k), shift));
+    for (uint32_t i0 = tid; i0 < n; i0 += 4 * kSelThreads) {
+      uint64_t k[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t i = i0 + u * kSelThreads;
+        k[u] = i < n ? err_key(err[i]) : ~0ull;  // ~0 matches no prefix
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if ((k[u] & mask) == prefix && k[u] != ~0ull) {
+          const int dgt = (int)((k[u] >> sh) & 0xff);
+          atomicAdd(&s_cnt[dgt], 1u);
+          atomicAdd(&s_sum[dgt], fixed_err(k[u], shift));
+        }
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      uint64_t cs = s_cumsum;
-      uint32_t cc = s_cumcnt;
-      int found = -1;
-      for (int b = 255; b >= 0; --b) {
-        if (s_cnt[b] == 0) continue;
-        if (cs + s_sum[b] >= need || (uint64_t)cc + s_cnt[b] >= kcap) {
-          found = b;
-          break;
-        }
-        cs += s_sum[b];
-        cc += s_cnt[b];
-      }
-      if (found < 0) found = 0;  // not reachable: the total meets need
+    // bins in decreasing digit order, thread t <-> digit 255 - t: running
+    // sums from the top digit down
+    const int bin = kSelThreads - 1 - tid;
+    uint64_t cs = s_sum[bin], cc = s_cnt[bin], tot_s, tot_c;
+    block_scan2(cs, cc, s_wa, s_wb, tot_s, tot_c);
+    if (level == 0) {
+      // the level-0 histogram holds every key: its sum is the total error
+      uint64_t need;
+      if (thr >= (double)tot_s) need = 1;  // at least one region is popped
+      else need = tot_s - (uint64_t)floor(thr);
+      if (tid == 0) s_need = need == 0 ? 1 : need;
+      __syncthreads();
+    }
+    const uint64_t need = s_need, cs0 = s_cumsum, cc0 = s_cumcnt;
+    // the pops end in the first non-empty bin (from the top) at which the
+    // error sum reaches need or the count reaches kcap; the test is
+    // monotone in the bin, so that bin is the last one that fails it
+    const bool hit = cs0 + cs >= need || cc0 + cc >= kcap;
+    const uint64_t hb = __ballot(hit);
+    if ((tid & 63) == 0) s_wa[tid >> 6] = (uint64_t)__popcll(hb);
+    __syncthreads();
+    const int nhit = (int)(s_wa[0] + s_wa[1] + s_wa[2] + s_wa[3]);
+    // thread of the found bin: the first one that hits (bin 0 if none does,
+    // which the total meeting need rules out)
+    const int tf = nhit > 0 ? kSelThreads - nhit : kSelThreads - 1;
+    if (tid == tf) {
+      const int found = kSelThreads - 1 - tf;
       s_prefix = prefix | ((uint64_t)found << sh);
       s_mask = mask | (0xffull << sh);
-      s_cumsum = cs;
-      s_cumcnt = cc;
+      s_cumsum = cs0 + cs - s_sum[found];
+      s_cumcnt = (uint32_t)(cc0 + cc - s_cnt[found]);
       s_lastcnt = s_cnt[found];
     }
     __syncthreads();
@@ -905,7 +946,8 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
   // regions carrying that key, lowest slot first.
   if (tid == 0) {
     uint64_t T = s_prefix;
-    unsigned long long fxT = (unsigned long long)ldexp(__longlong_as_double((long long)T), shift);
+    unsigned long long fxT = fixed_err(T, shift);
+    const uint64_t need = s_need;
     uint64_t rest = need > s_cumsum ? need - s_cumsum : 0;
     uint64_t r = fxT > 0 ? (rest + fxT - 1) / fxT : (uint64_t)s_lastcnt;
     if (r < 1) r = 1;
@@ -928,8 +970,6 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
       }
     }
     s_st = st;
-    s_tiecarry = 0;
-    s_selcarry = 0;
   }
   __syncthreads();
   if (s_flag) {
@@ -940,48 +980,86 @@ __global__ __launch_bounds__(256) void k_select(SelectLaunch L) {
     return;
   }
 
-  // 4. mark and cut in slot order; children go to the kind's list.
+  // 4. mark and cut in slot order; children go to the kind's list.  Each
+  //    thread marks four consecutive slots per chunk of 1024; the ranks among
+  //    the tied keys and among the selected regions come from block scans of
+  //    the per-thread counts (carried across chunks in registers), and the
+  //    chunk's selected slots are listed in LDS in rank order, so that
+  //    consecutive lanes cut consecutive regions (contiguous child slots)
+  //    and each cut reads its region's geometry before writing any.
+  __shared__ uint32_t s_selp[4 * kSelThreads];
   const uint64_t T = s_prefix;
   const uint32_t r = s_r;
   const int D = d.dim;
   EvalItem *list = L.lists[d.kind] + s_listbase;
-  for (uint32_t chunk = 0; chunk < n; chunk += 256) {
-    uint32_t i = chunk + tid;
-    uint64_t k = i < n ? err_key(err[i]) : 0;
-    bool tie = i < n && k == T;
-    uint32_t tot;
-    uint32_t trank = s_tiecarry + block_scan_bit(tie, s_wave, tot);
-    bool sel = i < n && (k > T || (tie && trank < r));
-    uint32_t tot2;
-    uint32_t at = s_selcarry + block_scan_bit(sel, s_wave, tot2);
-    if (sel) {
-      uint32_t p = d.base + i;
-      uint32_t q = d.base + n + at;
-      int sd = L.split[p];
-      double hd = L.geo[(kFieldH + sd) * S + p] * 0.5;
-      double vol = L.geo[kFieldVol * S + p];
-      double hq[kMaxDim];
-      for (int e = 0; e < D; ++e) {
-        double c = L.geo[(kFieldC + e) * S + p];
-        double h = (e == sd) ? hd : L.geo[(kFieldH + e) * S + p];
-        hq[e] = h;
-        L.geo[(kFieldC + e) * S + q] = (e == sd) ? c + hd : c;
-        L.geo[(kFieldH + e) * S + q] = h;
+  uint64_t tie_carry = 0, sel_carry = 0;
+  for (uint32_t chunk = 0; chunk < n; chunk += 4 * kSelThreads) {
+    const uint32_t i0 = chunk + 4 * tid;
+    uint64_t k[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) k[u] = i0 + u < n ? err_key(err[i0 + u]) : 0;
+    uint64_t nt = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) nt += (i0 + u < n && k[u] == T) ? 1 : 0;
+    // ties before this thread's, then the selections it makes
+    uint64_t tie_incl = nt, dummy = 0, tot_t, tot_d;
+    block_scan2(tie_incl, dummy, s_wa, s_wb, tot_t, tot_d);
+    uint64_t trank = tie_carry + tie_incl - nt;
+    uint32_t selm = 0;
+    uint64_t ns = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool in = i0 + u < n;
+      const bool tie = in && k[u] == T;
+      const bool sel = in && (k[u] > T || (tie && trank < r));
+      if (tie) ++trank;
+      selm |= sel ? 1u << u : 0u;
+      ns += sel ? 1 : 0;
+    }
+    uint64_t sel_incl = ns, dummy2 = 0, tot_s2, tot_d2;
+    block_scan2(sel_incl, dummy2, s_wa, s_wb, tot_s2, tot_d2);
+    uint32_t at = (uint32_t)(sel_incl - ns);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (selm >> u & 1u) s_selp[at++] = i0 + u;
+    __syncthreads();
+    for (uint32_t j = tid; j < (uint32_t)tot_s2; j += kSelThreads) {
+      const uint32_t p = d.base + s_selp[j];
+      const uint64_t a = sel_carry + j;
+      const uint32_t q = d.base + n + (uint32_t)a;
+      const int sd = L.split[p];
+      double c[kMaxDim], h[kMaxDim], hd = 0;
+#pragma unroll
+      for (int e = 0; e < kMaxDim; ++e) {
+        if (e < D) {
+          c[e] = L.geo[(kFieldC + e) * S + p];
+          h[e] = L.geo[(kFieldH + e) * S + p];
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < kMaxDim; ++e)
+        if (e < D && e == sd) hd = h[e] * 0.5;
+      const double vol = L.geo[kFieldVol * S + p];
+      double vq = 1;  // box_volume of the child, the same products in order
+#pragma unroll
+      for (int e = 0; e < kMaxDim; ++e) {
+        if (e >= D) continue;
+        const double he = (e == sd) ? hd : h[e];
+        vq *= 2 * he;
+        L.geo[(kFieldC + e) * S + q] = (e == sd) ? c[e] + hd : c[e];
+        L.geo[(kFieldH + e) * S + q] = he;
         if (e == sd) {
-          L.geo[(kFieldC + e) * S + p] = c - hd;
+          L.geo[(kFieldC + e) * S + p] = c[e] - hd;
           L.geo[(kFieldH + e) * S + p] = hd;
         }
       }
       L.geo[kFieldVol * S + p] = vol * 0.5;
-      L.geo[kFieldVol * S + q] = box_volume(D, hq);
-      list[2 * at] = EvalItem{p, (uint32_t)J};
-      list[2 * at + 1] = EvalItem{q, (uint32_t)J};
+      L.geo[kFieldVol * S + q] = vq;
+      list[2 * a] = EvalItem{p, (uint32_t)J};
+      list[2 * a + 1] = EvalItem{q, (uint32_t)J};
     }
-    __syncthreads();
-    if (tid == 0) {
-      s_tiecarry += tot;
-      s_selcarry += tot2;
-    }
+    tie_carry += tot_t;
+    sel_carry += tot_s2;
     __syncthreads();
   }
   if (tid == 0) {
@@ -1147,7 +1225,9 @@ int eval_kernel_ref(int kind, int prec, int variant, KernelRef *out) {
   return variant ? eval_ref_variant<1, PMix>(kind, out) : eval_ref_variant<0, PMix>(kind, out);
 }
 
-KernelRef select_kernel_ref() { return KernelRef{reinterpret_cast<const void *>(&k_select), 256u, 1u}; }
+KernelRef select_kernel_ref() {
+  return KernelRef{reinterpret_cast<const void *>(&k_select), (unsigned)kSelThreads, 1u};
+}
 KernelRef zero_kernel_ref() { return KernelRef{reinterpret_cast<const void *>(&k_zero), 256u, 1u}; }
 
 int eval_group_size(int kind, int variant) {
@@ -1173,7 +1253,7 @@ hipError_t launch_roots(const RootsLaunch &L, hipStream_t s) {
 
 hipError_t launch_select(const SelectLaunch &L, uint32_t njobs, hipStream_t s) {
   if (njobs == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_select, dim3(njobs), dim3(256), 0, s, L);
+  hipLaunchKernelGGL(k_select, dim3(njobs), dim3(kSelThreads), 0, s, L);
   return hipGetLastError();
 }
 
