@@ -635,10 +635,12 @@ int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
   size_t og = put(t->g, n1), op1 = put(t->p_lens1, n1),
          op2 = put(t->p_lens2 ? t->p_lens2 : t->p_lens1, n1), ow = put(t->w, n1),
          odw = put(t->dwdz, n1), orho = put(c->rho.data(), n1), ohmf = put(t->hmf, n2),
-         opl = put(t->P_lin, n2 + n1), obh = put(t->b_h, n2), oc = put(t->concentration, n2);
-  // P_lin keeps nb trailing zeros: the look-up has no upper guard
-  // (src/NNMap_Model.cu:826-839), so the last row's overrun reads zeros.
-  std::memset(h.data() + opl + n2, 0, n1 * sizeof(double));
+         opl = put(t->P_lin, n2);
+  // P_lin keeps nb trailing zeros (h is zero-filled; the caller's array has
+  // n2 entries): the look-up has no upper guard (src/NNMap_Model.cu:826-839),
+  // so the last row's overrun reads zeros.
+  put(nullptr, n1);
+  size_t obh = put(t->b_h, n2), oc = put(t->concentration, n2);
   if (hipMemcpy(c->tables.p, h.data(), total * sizeof(double), hipMemcpyHostToDevice) !=
       hipSuccess) {
     g3l_ctx_destroy(c);

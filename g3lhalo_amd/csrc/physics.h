@@ -230,8 +230,15 @@ G3L_HD int zix_round(const DevTables &T, double z) {
 G3L_HD int zix_trunc(const DevTables &T, double z) {
   return clampi((int)((z - T.zmin) * T.nb / (T.zmax - T.zmin)), T.nb - 1);
 }
+// Held in [-1, nb^2 + nb] before the conversion: an infinite k (a zero
+// comoving distance) would otherwise convert out of int's range, and
+// zi * nb + ki could overflow.  Every finite k in that range keeps its
+// index; beyond it the row-spill read lands past the padded table either
+// way and pk's clamp gives the same entry.  NaN maps to 0.
 G3L_HD int kix_round(const DevTables &T, double k) {
-  return (int)round(log(k / T.kmin) * T.nb / T.lkr);
+  const double x = round(log(k / T.kmin) * T.nb / T.lkr);
+  const double top = (double)T.nb * T.nb + T.nb;
+  return x >= 0 ? (int)fmin(x, top) : (x < 0 ? (int)fmax(x, -1.0) : 0);
 }
 // P(k) look-up, linear below kmin, no upper guard (src/NNMap_Model.cu:819-839):
 // an index past the row reads on into the next row; P_lin carries nb
