@@ -38,12 +38,13 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
 # WRITE_SIZE runs of this workload scaled by a copy kernel of known size
 # (profiles/k_eval_nnmap550_counters.md): 464.1 MB read + 43.1 MB written per
 # k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
-TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 507152728}
+TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 703011197}
 
 
 # Launch variant per kind when the autotuner does not run (the library's
 # built-in choice); the kernel's template argument is kind + 16 x variant.
-DEFAULT_VARIANT = {0: 0, 1: 1, 2: 2, 3: 2, 4: 1, 5: 1, 6: 2}
+# the library's built-in launch variant per kind (used when not autotuned)
+DEFAULT_VARIANT = {0: 0, 1: 0, 2: 3, 3: 2, 4: 0, 5: 3, 6: 2}
 
 
 def kernel_kv(tuned: dict, prec: str, kind: int) -> int:

@@ -141,8 +141,10 @@ struct g3l_ctx {
   DevBuf pgeo, psplit, plist, pjob, parg;
   HostBuf hgeo, hsplit;
   uint32_t pcap = 0;
-  // launch variant (block shape) per precision and integrand kind
-  int variant[2][kNumKinds] = {{0, 1, 2, 2, 1, 1, 2}, {0, 1, 2, 2, 1, 1, 2}};
+  // launch variant (block shape) per precision and integrand kind (n_bar,
+  // NNMap 1h/2h/3h, NMapMap 1h/2h/3h): g3l_autotune's choices on MI355X for
+  // the 550-triplet workloads, 11, 3 and 1 regions per block
+  int variant[2][kNumKinds] = {{0, 0, 3, 2, 0, 3, 2}, {0, 0, 3, 2, 0, 3, 2}};
   // refinement rounds of the batched engine and the server: G3L_LAUNCH_GRAPH
   // (one graph launch per round) or G3L_LAUNCH_STREAM (launches on the stream)
   int launch_mode = G3L_LAUNCH_GRAPH;

@@ -38,7 +38,7 @@ __device__ __forceinline__ double integrand(const DevTables &T, const TermArgs &
 // kind in its low four bits and a block-shape variant above them; kind_of_kv
 // and variant_of_kv take it apart.  Every variant computes the same values.
 constexpr int kVariantShift = 4;
-constexpr int kNumVariants = 3;
+constexpr int kNumVariants = 4;
 __host__ __device__ constexpr int kind_of_kv(int kv) { return kv & ((1 << kVariantShift) - 1); }
 __host__ __device__ constexpr int variant_of_kv(int kv) { return kv >> kVariantShift; }
 __host__ __device__ constexpr int make_kv(int kind, int v) { return kind | (v << kVariantShift); }
@@ -50,18 +50,25 @@ __host__ __device__ constexpr int make_kv(int kind, int v) { return kind | (v <<
 // SIMD.  Three shapes per dimension, picked per kind at run time
 // (g3l_autotune): variant 0 fills one block of up to 1024 threads, variants
 // 1 and 2 halve and quarter it so that more blocks share a compute unit and
-// one block's barriers overlap another's work:
+// one block's barriers overlap another's work, and variant 3 is the
+// group size whose points fill their waves best:
 //   n_bar  (1-D): 68 regions x 15 points = 1020 threads (all variants);
-//   1-halo (5-D): 11 x 93 = 1023 points, 5 x 93 = 465 of 512, 2 x 93 = 186 of 192;
-//   2-halo (6-D):  4 x 149 = 596 points of 640 (u_NFW pass 4 x 318 = 1272 in
-//                  two rounds), 2 x 149 = 298 of 320, 1 x 149 of 192;
-//   3-halo (7-D):  4 x 241 = 964 points, 2 x 241 = 482 of 512, 1 x 241 of 256.
+//   1-halo (5-D): 11 x 93 = 1023 points, 5 x 93 = 465 of 512, 2 x 93 = 186 of 192,
+//                 4 x 93 = 372 of 384;
+//   2-halo (6-D):  6 x 149 = 894 points of 896 (u_NFW pass 6 x 318 = 1908 in
+//                  three rounds), 2 x 149 = 298 of 320, 1 x 149 of 192,
+//                  3 x 149 = 447 of 448;
+//   3-halo (7-D):  4 x 241 = 964 points, 2 x 241 = 482 of 512, 1 x 241 of 256,
+//                  3 x 241 = 723 of 768.
 template <int KV>
 struct KGroup {
   static constexpr int D = kind_dim(kind_of_kv(KV));
   static constexpr int V = variant_of_kv(KV);
   static constexpr int P = rule_points(D);
-  static constexpr int G = D == 1 ? 68 : D == 5 ? (V == 2 ? 2 : V ? 5 : 11) : (V == 2 ? 1 : V ? 2 : 4);
+  static constexpr int G = D == 1   ? 68
+                           : D == 5 ? (V == 3 ? 4 : V == 2 ? 2 : V ? 5 : 11)
+                           : D == 6 ? (V == 3 ? 3 : V == 2 ? 1 : V ? 2 : 6)
+                                    : (V == 3 ? 3 : V == 2 ? 1 : V ? 2 : 4);
   static constexpr int NT = (G * P + 63) / 64 * 64;
   static_assert(NT <= 1024, "one block holds every point of its regions");
 };
@@ -1175,9 +1182,11 @@ hipError_t launch_eval(int kind, int prec, int variant, const EvalLaunch &L, hip
   if (kind == kNbar) return launch_eval_kind<kNbar, P64>(L, s);
   if (variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
   if (prec == 0) {
+    if (variant == 3) return launch_eval_variant<3, P64>(kind, L, s);
     if (variant == 2) return launch_eval_variant<2, P64>(kind, L, s);
     return variant ? launch_eval_variant<1, P64>(kind, L, s) : launch_eval_variant<0, P64>(kind, L, s);
   }
+  if (variant == 3) return launch_eval_variant<3, PMix>(kind, L, s);
   if (variant == 2) return launch_eval_variant<2, PMix>(kind, L, s);
   return variant ? launch_eval_variant<1, PMix>(kind, L, s) : launch_eval_variant<0, PMix>(kind, L, s);
 }
@@ -1218,9 +1227,11 @@ int eval_kernel_ref(int kind, int prec, int variant, KernelRef *out) {
   }
   if (variant < 0 || variant >= kNumVariants) return -1;
   if (prec == 0) {
+    if (variant == 3) return eval_ref_variant<3, P64>(kind, out);
     if (variant == 2) return eval_ref_variant<2, P64>(kind, out);
     return variant ? eval_ref_variant<1, P64>(kind, out) : eval_ref_variant<0, P64>(kind, out);
   }
+  if (variant == 3) return eval_ref_variant<3, PMix>(kind, out);
   if (variant == 2) return eval_ref_variant<2, PMix>(kind, out);
   return variant ? eval_ref_variant<1, PMix>(kind, out) : eval_ref_variant<0, PMix>(kind, out);
 }
@@ -1234,6 +1245,10 @@ int eval_group_size(int kind, int variant) {
   const int d = kind_dim(kind);
   if (d == 1) return KGroup<kNbar>::G;
   const int k = d == 5 ? kNN1 : d == 6 ? kNN2 : kNN3;
+  if (variant == 3) {
+    return k == kNN1 ? KGroup<make_kv(kNN1, 3)>::G
+                     : k == kNN2 ? KGroup<make_kv(kNN2, 3)>::G : KGroup<make_kv(kNN3, 3)>::G;
+  }
   if (variant == 2) {
     return k == kNN1 ? KGroup<make_kv(kNN1, 2)>::G
                      : k == kNN2 ? KGroup<make_kv(kNN2, 2)>::G : KGroup<make_kv(kNN3, 2)>::G;
