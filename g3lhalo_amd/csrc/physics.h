@@ -748,10 +748,12 @@ template <int KIND>
 struct Spec;
 template <>
 struct Spec<kNN1> {
-  static constexpr int D = 5, NM = 1, NU = 2, nss_slot = 0;
+  static constexpr int D = 5, NM = 1, NU = 3, nss_slot = 0;
   static constexpr bool two_hods = true, has_bilin = false, has_pk = false;
   static constexpr int mdim[3] = {3, 0, 0};
-  static constexpr USpec u[6] = {{kL1 | bit(3) | bit(4), 1, 0, 1}, {kL2 | bit(3) | bit(4), 2, 0, 2}};
+  static constexpr USpec u[6] = {{kL1 | bit(3) | bit(4), 1, 0, 1},   // u(k1, m, f1)
+                                 {kL2 | bit(3) | bit(4), 2, 0, 2},   // u(k2, m, f2)
+                                 {kL3 | bit(3) | bit(4), 3, 0, 0}};  // u(k3, m), every point
 };
 template <>
 struct Spec<kNN2> {
@@ -776,11 +778,12 @@ struct Spec<kNN3> {
 };
 template <>
 struct Spec<kNM1> {
-  static constexpr int D = 5, NM = 1, NU = 2, nss_slot = -1;
+  static constexpr int D = 5, NM = 1, NU = 3, nss_slot = -1;
   static constexpr bool two_hods = false, has_bilin = false, has_pk = false;
   static constexpr int mdim[3] = {3, 0, 0};
   static constexpr USpec u[6] = {{kL1 | bit(3) | bit(4), 1, 0, 1},   // u(k1, m, f)
-                                 {kL2 | bit(3) | bit(4), 2, 0, 0}};  // u(k2, m)
+                                 {kL2 | bit(3) | bit(4), 2, 0, 0},   // u(k2, m)
+                                 {kL3 | bit(3) | bit(4), 3, 0, 0}};  // u(k3, m), every point
 };
 template <>
 struct Spec<kNM2> {
@@ -804,11 +807,10 @@ struct Spec<kNM3> {
                                  {kL1 | bit(4) | bit(6), 1, 1, 1}};  // u(k1, m2, f)
 };
 // Whether the integrand of KIND needs u_NFW's normalisation (c, ln(1+c), F)
-// for mass slot `slot` with f selector `fsel`: its partial u_NFW factors, and
-// the 1-halo terms' per-point u_NFW(k3, m) with f = 1.
+// for mass slot `slot` with f selector `fsel`: one of its u_NFW factors uses
+// it.
 template <int KIND>
 constexpr bool uses_nfw(int slot, int fsel) {
-  if ((KIND == kNN1 || KIND == kNM1) && slot == 0 && fsel == 0) return true;
   for (int i = 0; i < Spec<KIND>::NU; ++i)
     if (Spec<KIND>::u[i].slot == slot && Spec<KIND>::u[i].fsel == fsel) return true;
   return false;
@@ -939,7 +941,7 @@ G3L_HD double nnmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   const int zi = G.zr;
   Occ o1 = s.occ1(0), o2 = s.occ2(0);
   double nss = s.nss(0);
-  double u3 = P::u(s.nfw(0, 0), G.k3);
+  double u3 = s.template u<2>();
   double u1 = s.template u<0>();
   double u2 = s.template u<1>();
   double m = M.m, l1 = G.l1, l2 = G.l2;
@@ -1008,11 +1010,10 @@ G3L_HD double nmapmap_1h(const DevTables &T, const TermArgs &a, const S &s) {
   const int zi = G.zr;
   double nbar = a.nb1[zi];
   Occ o = s.occ1(0);
-  Nfw n1 = s.nfw(0, 0);
   double uf = s.template u<0>();
   double m = M.m, l1 = G.l1, l2 = G.l2;
   double B = s.zpre() * T.hmf[zi * T.nb + M.ir] * m * m * (o.nc + o.ns * uf) *
-             s.template u<1>() * P::u(n1, G.k3);
+             s.template u<1>() * s.template u<2>();
   double b = s.proj(B);
   return l1 * l2 * m * l1 * l2 * s.ap1() * s.ap2() * s.ap3() * b / nbar;
 }

@@ -16,8 +16,9 @@
   (every index is clamped).  The tables have 64 bins: at 32 the root
   region's centre mass, log10 m = 10 + 3.390625, sits on an exact rounding
   tie of the mass index (3.390625 x 32 / 7 = 15.5), which the last bit of
-  10^x decides, so the oracle's pow(10, x) and the FP64 policy's exp10 take
-  neighbouring bins there and refine differently from the first split on.
+  pow(10, x) decides; the device's pow and the host library's differ in that
+  bit there, so the two take neighbouring bins and refine differently from
+  the first split on.
 * An empty batch.
 """
 import numpy as np
