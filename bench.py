@@ -38,7 +38,7 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
 # WRITE_SIZE runs of this workload scaled by a copy kernel of known size
 # (profiles/k_eval_nnmap550_counters.md): 464.1 MB read + 43.1 MB written per
 # k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
-TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 702964518}
+TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 696404502}
 
 
 # Launch variant per kind when the autotuner does not run (the library's

@@ -47,17 +47,17 @@ __host__ __device__ constexpr int make_kv(int kind, int v) { return kind | (v <<
 // item per round, in straight-line code -- no loop for the compiler to hoist
 // the FP64 constants of the special functions out of, which otherwise pins
 // them in registers for the whole kernel and drops occupancy to one wave per
-// SIMD.  Three shapes per dimension, picked per kind at run time
+// SIMD.  Four shapes per dimension, picked per kind at run time
 // (g3l_autotune): variant 0 fills one block of up to 1024 threads, variants
-// 1 and 2 halve and quarter it so that more blocks share a compute unit and
-// one block's barriers overlap another's work, and variant 3 is the
-// group size whose points fill their waves best:
+// 1 and 2 take fewer regions so that more blocks share a compute unit and
+// one block's barriers overlap another's work, and variant 3 is a group
+// size whose points fill their waves closely:
 //   n_bar  (1-D): 68 regions x 15 points = 1020 threads (all variants);
 //   1-halo (5-D): 11 x 93 = 1023 points, 5 x 93 = 465 of 512, 2 x 93 = 186 of 192,
 //                 4 x 93 = 372 of 384;
 //   2-halo (6-D):  6 x 149 = 894 points of 896 (u_NFW pass 6 x 318 = 1908 in
 //                  three rounds), 2 x 149 = 298 of 320, 1 x 149 of 192,
-//                  3 x 149 = 447 of 448;
+//                  3 x 149 = 447 of 512;
 //   3-halo (7-D):  4 x 241 = 964 points, 2 x 241 = 482 of 512, 1 x 241 of 256,
 //                  3 x 241 = 723 of 768.
 template <int KV>
@@ -69,7 +69,9 @@ struct KGroup {
                            : D == 5 ? (V == 3 ? 4 : V == 2 ? 2 : V ? 5 : 11)
                            : D == 6 ? (V == 3 ? 3 : V == 2 ? 1 : V ? 2 : 6)
                                     : (V == 3 ? 3 : V == 2 ? 1 : V ? 2 : 4);
-  static constexpr int NT = (G * P + 63) / 64 * 64;
+  // the 2-halo kinds at three regions take 512 threads rather than 448, so
+  // that their 954 u_NFW items fit two per thread
+  static constexpr int NT = D == 6 && V == 3 ? 512 : (G * P + 63) / 64 * 64;
   static_assert(NT <= 1024, "one block holds every point of its regions");
 };
 
