@@ -161,7 +161,7 @@ typedef struct g3l_tune_report {
   int32_t group[8][4];
 } g3l_tune_report;
 int g3l_autotune(g3l_ctx *ctx, int32_t hodset, int32_t precision, g3l_tune_report *out);
-/* Sets one kind's variant directly (kind 1-6). */
+/* Sets one kind's variant directly (kind 1-6, variant 0-3). */
 int g3l_set_variant(g3l_ctx *ctx, int32_t kind, int32_t precision, int32_t variant);
 
 /* How the refinement rounds of g3l_eval_batch and of the servers made on
