@@ -931,8 +931,9 @@ __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
     }
     const uint64_t need = s_need, cs0 = s_cumsum, cc0 = s_cumcnt;
     // the pops end in the first non-empty bin (from the top) at which the
-    // error sum reaches need or the count reaches kcap; the test is
-    // monotone in the bin, so that bin is the last one that fails it
+    // error sum reaches need or the count reaches kcap; the running sums only
+    // grow in thread order, so the threads that pass the test are a suffix
+    // and the found bin is its first
     const bool hit = cs0 + cs >= need || cc0 + cc >= kcap;
     const uint64_t hb = __ballot(hit);
     if ((tid & 63) == 0) s_wa[tid >> 6] = (uint64_t)__popcll(hb);
