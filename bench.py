@@ -40,6 +40,22 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
 # k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
 TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 696404502}
 
+# Hardware FP64 flops per evaluation, (kind, launch variant) -> 2 FMA + ADD +
+# MUL + TRANS of the SQ_INSTS_VALU_*_F64 counters x 64 lanes / evaluations,
+# from the same profile: the work the kernel executes, against the credited
+# algorithmic work of flops.FLOPS_PER_EVAL.
+HW_FP64_FLOPS_PER_EVAL = {(2, 3): 1589, (3, 2): 1111}
+
+# Algorithmic HBM bytes per region an evaluation launch handles: the
+# region's centre, half-widths and volume (2D + 1 doubles) and its list item
+# read; its value, error and split byte written.
+def region_bytes(dim: int) -> int:
+    return (2 * dim + 1) * 8 + 8 + 2 * 8 + 1
+
+
+KIND_DIM = {0: 1, 1: 5, 2: 6, 3: 7, 4: 5, 5: 6, 6: 7}
+KIND_PTS = {1: 15, 5: 93, 6: 149, 7: 241}
+
 
 # Launch variant per kind when the autotuner does not run (the library's
 # built-in choice); the kernel's template argument is kind + 16 x variant.
@@ -417,6 +433,19 @@ def main(argv=None):
     dom_flops = float(kind_evals[dom]) * flops.FLOPS_PER_EVAL[dom]
     achieved = dom_flops / (kind_ms[dom] * 1e-3) / 1e12 if kind_ms[dom] > 0 else 0.0
     peak = FP64_VALU_PEAK_TFLOPS
+    dom_var = int(tuned["fp64"][dom]["variant"]) if "fp64" in tuned else DEFAULT_VARIANT[dom]
+    hw_fpe = HW_FP64_FLOPS_PER_EVAL.get((dom, dom_var))
+    hw_fp64 = None
+    if hw_fpe is not None and kind_ms[dom] > 0:
+        hw_t = float(kind_evals[dom]) * hw_fpe / (kind_ms[dom] * 1e-3) / 1e12
+        hw_fp64 = {"flops_per_eval": hw_fpe, "tflops": round(hw_t, 3),
+                   "frac": round(hw_t / peak, 4),
+                   "source": "SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 counters of this kernel "
+                             "(profiles/k_eval_nnmap550_counters.md)"}
+    dim = KIND_DIM[dom]
+    regions_per_launch = (float(kind_evals[dom]) / KIND_PTS[dim] / kind_launches[dom]
+                          if kind_launches[dom] else 0.0)
+    alg_bytes = int(round(regions_per_launch * region_bytes(dim)))
     mixed_info = None
     if mixed is not None:
         mdev = np.array([abs(m.value - r.value) / max(abs(r.value), 1e-300)
@@ -441,7 +470,10 @@ def main(argv=None):
                                    f"({KIND_NAMES[mk]})",
                          "achieved": round(mach, 3), "peak": round(flops.mixed_peak_tflops(mk), 2),
                          "unit": "TFLOP/s",
-                         "frac": round(mach / flops.mixed_peak_tflops(mk), 4)},
+                         "frac": round(mach / flops.mixed_peak_tflops(mk), 4),
+                         "peak_note": "FP32 at the rate of the unpacked instructions emitted "
+                                      "(78.6 TFLOP/s, as FP64); with packed FP32 the ceiling "
+                                      f"would be {flops.mixed_peak_tflops(mk, True):.1f}"},
             "note": "FP32 transcendental bulk; not the headline: these integrals end at the "
                     "evaluation cap unconverged, and FP32 noise re-orders near-tied regions"}
 
@@ -476,12 +508,17 @@ def main(argv=None):
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
                        "p99": round(float(np.percentile(step_ms, 99)), 3),
                        "note": "batched form: every integral of a step completes with the step"},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
-                     "compute": "FP64 vector ALU: no contraction maps onto MFMA; MI355X's "
-                                "FP64 vector and matrix peaks are both 78.6 TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "compute": "FP64 vector ALU issue (no MFMA: no contraction); MI355X FP64 "
+                                "vector peak 78.6 TFLOP/s",
                      "peak": round(peak, 2), "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),
+                     "hw_fp64": hw_fp64,
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((a.workload, dom)),
+                     "algorithmic_bytes": alg_bytes,
+                     "traffic_over_algorithmic": (
+                         round(TRAFFIC_BYTES_PER_LAUNCH[(a.workload, dom)] / alg_bytes, 2)
+                         if (a.workload, dom) in TRAFFIC_BYTES_PER_LAUNCH and alg_bytes else None),
                      "kernel": f"k_eval<{kernel_kv(tuned, 'fp64', dom)},P64> "
                                f"({KIND_NAMES[dom]}, fused points + integrand + rule)",
                      "avg_launch_ms": round(float(kind_ms[dom] / max(kind_launches[dom], 1)), 4),

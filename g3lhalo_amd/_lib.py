@@ -106,6 +106,9 @@ SIGNATURES = {
     "g3l_integrand": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int32, C.c_size_t, _dp,
                                 _dp]),
     "g3l_last_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "g3l_set_trace": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "g3l_last_trace": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.c_uint32,
+                                 C.POINTER(C.c_uint32)]),
     "g3l_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneReport)]),
     "g3l_set_variant": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "g3l_set_launch_mode": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -239,6 +242,19 @@ class Context:
         out = (Result * n)()
         check("g3l_eval_batch", load().g3l_eval_batch(self.h, arr, n, int(precision), out))
         return list(out)
+
+    def set_trace(self, cap: int):
+        """Record the batch-size trace of following eval_batch calls (cap
+        rounds per integral; 0 stops)."""
+        check("g3l_set_trace", load().g3l_set_trace(self.h, int(cap)))
+
+    def last_trace(self, i: int, cap: int = 4097) -> np.ndarray:
+        """Regions per integrand call of integral i of the last traced
+        eval_batch (the root region first)."""
+        buf = (C.c_uint32 * cap)()
+        n = C.c_uint32()
+        check("g3l_last_trace", load().g3l_last_trace(self.h, int(i), buf, int(cap), C.byref(n)))
+        return np.array(buf[:n.value], dtype=np.int64)
 
     def integrand(self, req: Request, x, precision=FP64):
         x = np.ascontiguousarray(x, dtype=np.float64)

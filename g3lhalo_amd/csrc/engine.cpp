@@ -157,54 +157,23 @@ int ctx_create_bare(g3l_ctx **out) {
 // Batched engine: all jobs refine together; per round one selection launch
 // and one evaluation launch per integrand kind.
 
-int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed, int prec) {
-  const uint32_t nj = (uint32_t)E.desc.size();
-  out.assign(nj, JobState{});
-  if (nj == 0) return G3L_OK;
-  auto t0 = std::chrono::steady_clock::now();
-  uint64_t slots = 0;
-  uint64_t per_kind[kNumKinds] = {0};
-  for (auto &d : E.desc) {
-    d.base = (uint32_t)slots;
-    slots += d.cap;
-    per_kind[d.kind] += 2ull * d.cap;
-  }
-  if (slots >= (1ull << 32) - 1) return G3L_ENOMEM;
-  uint64_t list_total = 0;
-  for (int k = 0; k < kNumKinds; ++k) list_total += per_kind[k];
-  int rc;
-  if ((rc = c->pool.ensure(slots * kNumFields * sizeof(double)))) return rc;
-  if ((rc = c->split.ensure(slots))) return rc;
-  if ((rc = c->lists.ensure(list_total * sizeof(EvalItem)))) return rc;
-  if ((rc = c->jobs.ensure(nj * sizeof(JobDesc)))) return rc;
-  if ((rc = c->state.ensure(nj * sizeof(JobState)))) return rc;
-  if ((rc = c->args.ensure(E.args.size() * sizeof(TermArgs)))) return rc;
-  if ((rc = c->boxes.ensure(E.boxes.size() * sizeof(double)))) return rc;
-  if ((rc = c->counters.ensure(64))) return rc;
-  if ((rc = c->hcounters.ensure(64))) return rc;
+int refine_jobs(g3l_ctx *c, const EngineBufs &B, uint32_t nj, bool timed, int prec,
+                bool use_graph) {
   hipStream_t s = c->stream;
-  G3L_HIP(hipMemcpyAsync(c->jobs.p, E.desc.data(), nj * sizeof(JobDesc), hipMemcpyHostToDevice, s));
-  G3L_HIP(hipMemcpyAsync(c->args.p, E.args.data(), E.args.size() * sizeof(TermArgs),
-                         hipMemcpyHostToDevice, s));
-  G3L_HIP(hipMemcpyAsync(c->boxes.p, E.boxes.data(), E.boxes.size() * sizeof(double),
-                         hipMemcpyHostToDevice, s));
+  uint32_t *dcnt = B.dcnt;
+  uint32_t *hcnt = B.hcnt;
   EvalItem *lists[kNumKinds];
-  {
-    uint64_t off = 0;
-    for (int k = 0; k < kNumKinds; ++k) {
-      lists[k] = c->lists.as<EvalItem>() + off;
-      off += per_kind[k];
-    }
-  }
-  uint32_t *dcnt = c->counters.as<uint32_t>();
-  uint32_t *hcnt = c->hcounters.as<uint32_t>();
+  for (int k = 0; k < kNumKinds; ++k) lists[k] = B.lists[k];
+  const uint64_t *per_kind = B.per_kind;
+  const uint64_t slots = B.slots;
+  int rc;
 
   RootsLaunch R{};
-  R.jobs = c->jobs.as<JobDesc>();
-  R.boxes = c->boxes.as<double>();
-  R.state = c->state.as<JobState>();
-  R.geo = c->pool.as<double>();
-  R.split = c->split.as<uint8_t>();
+  R.jobs = B.jobs;
+  R.boxes = B.boxes;
+  R.state = B.state;
+  R.geo = B.geo;
+  R.split = B.split;
   R.stride = slots;
   for (int k = 0; k < kNumKinds; ++k) R.lists[k] = lists[k];
   R.counters = dcnt;
@@ -214,24 +183,26 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
 
   EvalLaunch EL{};
   EL.T = c->T;
-  EL.args = c->args.as<TermArgs>();
-  EL.jobs = c->jobs.as<JobDesc>();
-  EL.geo = c->pool.as<double>();
-  EL.split = c->split.as<uint8_t>();
+  EL.args = B.args;
+  EL.jobs = B.jobs;
+  EL.geo = B.geo;
+  EL.split = B.split;
   EL.stride = slots;
   EL.njobs = nj;
 
   SelectLaunch SL{};
-  SL.jobs = c->jobs.as<JobDesc>();
-  SL.state = c->state.as<JobState>();
-  SL.geo = c->pool.as<double>();
-  SL.split = c->split.as<uint8_t>();
+  SL.jobs = B.jobs;
+  SL.state = B.state;
+  SL.geo = B.geo;
+  SL.split = B.split;
   SL.stride = slots;
   for (int k = 0; k < kNumKinds; ++k) {
     SL.lists[k] = lists[k];
     SL.list_cap[k] = (uint32_t)std::min<uint64_t>(per_kind[k], 0xffffffffull);
   }
   SL.counters = dcnt;
+  SL.trace = B.trace;
+  SL.trace_cap = B.trace ? B.trace_cap : 0;
 
   double eval_ms = 0, select_ms = 0;
   double kind_ms[kNumKinds] = {0};
@@ -243,19 +214,18 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
   G3L_HIP(hipStreamSynchronize(s));
   // graph mode: the round's launches and the counter copy are one graph
   // (every node records its events; they are read only when timed)
-  const bool use_graph = c->launch_mode == G3L_LAUNCH_GRAPH;
   RoundGraph &graph = c->graph;
   if (use_graph) {
     // the executable graph is kept while the kinds, variants, precision,
     // job count and buffers stay the same (repeated calls of one workload)
     bool in_run[kNumKinds];
     const int *var = c->variant[prec == G3L_MIXED];
-    std::vector<uint64_t> key = {(uint64_t)prec, nj, slots, (uint64_t)(uintptr_t)c->pool.p,
-                                 (uint64_t)(uintptr_t)c->lists.p, (uint64_t)(uintptr_t)c->jobs.p,
-                                 (uint64_t)(uintptr_t)c->state.p, (uint64_t)(uintptr_t)c->args.p,
-                                 (uint64_t)(uintptr_t)c->split.p,
-                                 (uint64_t)(uintptr_t)c->counters.p,
-                                 (uint64_t)(uintptr_t)c->hcounters.p};
+    std::vector<uint64_t> key = {(uint64_t)prec, nj, slots, (uint64_t)(uintptr_t)B.geo,
+                                 (uint64_t)(uintptr_t)lists[0], (uint64_t)(uintptr_t)B.jobs,
+                                 (uint64_t)(uintptr_t)B.state, (uint64_t)(uintptr_t)B.args,
+                                 (uint64_t)(uintptr_t)B.split, (uint64_t)(uintptr_t)dcnt,
+                                 (uint64_t)(uintptr_t)hcnt, (uint64_t)(uintptr_t)SL.trace,
+                                 SL.trace_cap};
     for (int k = 0; k < kNumKinds; ++k) {
       in_run[k] = per_kind[k] != 0;
       key.push_back(per_kind[k]);
@@ -313,26 +283,95 @@ int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed
     ++rounds;
     if (hcnt[kNumKinds] == 0) break;
   }
-  G3L_HIP(hipMemcpyAsync(out.data(), c->state.p, nj * sizeof(JobState), hipMemcpyDeviceToHost, s));
-  G3L_HIP(hipStreamSynchronize(s));
-  uint64_t evals = 0;
-  uint64_t kind_evals[kNumKinds] = {0};
-  for (uint32_t j = 0; j < nj; ++j) {
-    evals += out[j].neval;
-    kind_evals[E.desc[j].kind] += out[j].neval;
-  }
-  auto t1 = std::chrono::steady_clock::now();
   c->last.eval_ms = eval_ms;
   c->last.select_ms = select_ms;
-  c->last.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   c->last.eval_launches = launches;
-  c->last.evals = evals;
   c->last.rounds = rounds;
   for (int k = 0; k < 8; ++k) {
     c->last.kind_ms[k] = k < kNumKinds ? kind_ms[k] : 0;
     c->last.kind_launches[k] = k < kNumKinds ? kind_launches[k] : 0;
-    c->last.kind_evals[k] = k < kNumKinds ? kind_evals[k] : 0;
   }
+  return G3L_OK;
+}
+
+void finish_stats(g3l_ctx *c, const JobDesc *desc, const JobState *st, uint32_t nj,
+                  double wall_ms) {
+  uint64_t evals = 0;
+  uint64_t kind_evals[kNumKinds] = {0};
+  for (uint32_t j = 0; j < nj; ++j) {
+    evals += st[j].neval;
+    kind_evals[desc[j].kind] += st[j].neval;
+  }
+  c->last.wall_ms = wall_ms;
+  c->last.evals = evals;
+  for (int k = 0; k < 8; ++k) c->last.kind_evals[k] = k < kNumKinds ? kind_evals[k] : 0;
+}
+
+int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed, int prec,
+               bool trace) {
+  const uint32_t nj = (uint32_t)E.desc.size();
+  out.assign(nj, JobState{});
+  if (nj == 0) return G3L_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  EngineBufs B{};
+  for (auto &d : E.desc) {
+    d.base = (uint32_t)B.slots;
+    B.slots += d.cap;
+    B.per_kind[d.kind] += 2ull * d.cap;
+  }
+  if (B.slots >= (1ull << 32) - 1) return G3L_ENOMEM;
+  uint64_t list_total = 0;
+  for (int k = 0; k < kNumKinds; ++k) list_total += B.per_kind[k];
+  int rc;
+  if ((rc = c->pool.ensure(B.slots * kNumFields * sizeof(double)))) return rc;
+  if ((rc = c->split.ensure(B.slots))) return rc;
+  if ((rc = c->lists.ensure(list_total * sizeof(EvalItem)))) return rc;
+  if ((rc = c->jobs.ensure(nj * sizeof(JobDesc)))) return rc;
+  if ((rc = c->state.ensure(nj * sizeof(JobState)))) return rc;
+  if ((rc = c->args.ensure(E.args.size() * sizeof(TermArgs)))) return rc;
+  if ((rc = c->boxes.ensure(E.boxes.size() * sizeof(double)))) return rc;
+  if ((rc = c->counters.ensure(64))) return rc;
+  if ((rc = c->hcounters.ensure(64))) return rc;
+  hipStream_t s = c->stream;
+  if (trace && c->trace_cap > 0) {
+    const size_t entries = (size_t)nj * c->trace_cap;
+    if (entries > 0x7fffffffull) return G3L_EINVAL;
+    if ((rc = c->trace.ensure(entries * sizeof(uint32_t)))) return rc;
+    G3L_HIP(launch_zero(c->trace.as<uint32_t>(), (int)entries, s));
+    B.trace = c->trace.as<uint32_t>();
+    B.trace_cap = c->trace_cap;
+  }
+  G3L_HIP(hipMemcpyAsync(c->jobs.p, E.desc.data(), nj * sizeof(JobDesc), hipMemcpyHostToDevice, s));
+  G3L_HIP(hipMemcpyAsync(c->args.p, E.args.data(), E.args.size() * sizeof(TermArgs),
+                         hipMemcpyHostToDevice, s));
+  G3L_HIP(hipMemcpyAsync(c->boxes.p, E.boxes.data(), E.boxes.size() * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  {
+    uint64_t off = 0;
+    for (int k = 0; k < kNumKinds; ++k) {
+      B.lists[k] = c->lists.as<EvalItem>() + off;
+      off += B.per_kind[k];
+    }
+  }
+  B.geo = c->pool.as<double>();
+  B.split = c->split.as<uint8_t>();
+  B.jobs = c->jobs.as<JobDesc>();
+  B.state = c->state.as<JobState>();
+  B.args = c->args.as<TermArgs>();
+  B.boxes = c->boxes.as<double>();
+  B.dcnt = c->counters.as<uint32_t>();
+  B.hcnt = c->hcounters.as<uint32_t>();
+  if ((rc = refine_jobs(c, B, nj, timed, prec, c->launch_mode == G3L_LAUNCH_GRAPH))) return rc;
+  G3L_HIP(hipMemcpyAsync(out.data(), B.state, nj * sizeof(JobState), hipMemcpyDeviceToHost, s));
+  G3L_HIP(hipStreamSynchronize(s));
+  if (B.trace) {
+    c->trace_jobs = nj;
+    c->trace_rounds.resize(nj);
+    for (uint32_t j = 0; j < nj; ++j) c->trace_rounds[j] = out[j].rounds;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  finish_stats(c, E.desc.data(), out.data(), nj,
+               std::chrono::duration<double, std::milli>(t1 - t0).count());
   return G3L_OK;
 }
 
@@ -668,6 +707,7 @@ int g3l_ctx_destroy(g3l_ctx *c) {
   c->pool.release(); c->split.release(); c->lists.release(); c->jobs.release();
   c->state.release(); c->args.release(); c->boxes.release(); c->counters.release();
   c->hcounters.release();
+  c->trace.release();
   c->pgeo.release(); c->psplit.release(); c->plist.release(); c->pjob.release();
   c->parg.release(); c->hgeo.release(); c->hsplit.release();
   for (auto &e : c->ev)
@@ -790,7 +830,7 @@ int g3l_eval_batch(g3l_ctx *c, const g3l_request *reqs, size_t n, int32_t precis
     E.boxes.insert(E.boxes.end(), box, box + 2 * kMaxDim);
   }
   std::vector<JobState> st;
-  int rc = run_engine(c, E, st, true, precision);
+  int rc = run_engine(c, E, st, true, precision, true);
   if (rc) return rc;
   for (size_t i = 0; i < n; ++i) {
     out[i].raw = st[i].val;
@@ -841,6 +881,29 @@ int g3l_integrand(g3l_ctx *c, const g3l_request *r, int32_t precision, size_t np
     rc = G3L_EHIP;
   dx.release(); df.release(); da.release();
   return rc;
+}
+
+int g3l_set_trace(g3l_ctx *c, uint32_t cap) {
+  if (!c || cap > 4096) return G3L_EINVAL;
+  c->trace_cap = cap;
+  c->trace_jobs = 0;
+  c->trace_rounds.clear();
+  return G3L_OK;
+}
+
+int g3l_last_trace(g3l_ctx *c, size_t i, uint32_t *regions, uint32_t cap, uint32_t *len) {
+  if (!c || !len || (cap && !regions)) return G3L_EINVAL;
+  if (c->trace_cap == 0 || i >= c->trace_jobs) return G3L_EINVAL;
+  const uint32_t rounds = c->trace_rounds[i] + 1;  // the root's evaluation included
+  uint32_t n = std::min<uint32_t>(std::min<uint32_t>(rounds, c->trace_cap + 1), cap);
+  if (n > 0) {
+    regions[0] = 1;
+    if (n > 1)
+      G3L_HIP(hipMemcpy(regions + 1, c->trace.as<uint32_t>() + i * c->trace_cap,
+                        (n - 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  }
+  *len = n;
+  return G3L_OK;
 }
 
 int g3l_last_batch_stats(g3l_ctx *c, g3l_batch_stats *out) {

@@ -153,6 +153,12 @@ struct g3l_ctx {
   hipEvent_t ev[4]{};
   hipEvent_t kev[2 * kNumKinds]{};
   g3l_batch_stats last{};
+  // batch-size trace of the last g3l_eval_batch (g3l_set_trace): per job,
+  // trace_cap entries of regions per selection round, and its round count
+  DevBuf trace;
+  uint32_t trace_cap = 0;
+  uint32_t trace_jobs = 0;
+  std::vector<uint32_t> trace_rounds;
 };
 
 namespace g3l {
@@ -167,8 +173,40 @@ struct EngineJobs {
   std::vector<double> boxes;
 };
 // Refines every job to convergence or its budget on c's device buffers and
-// stream; out receives each job's final state (engine.cpp).
-int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed, int prec);
+// stream; out receives each job's final state (engine.cpp).  trace records
+// the batch-size trace when g3l_set_trace enabled it.
+int run_engine(g3l_ctx *c, EngineJobs &E, std::vector<JobState> &out, bool timed, int prec,
+               bool trace = false);
+
+// Device buffers of one batched run: region pool (slots slots, field-major),
+// split bytes, one evaluation list per kind (per_kind entries each), job
+// descriptors, states, integrand parameters and boxes, the round counters
+// (dcnt on the device, hcnt readable by the host after a stream
+// synchronisation) and an optional trace (SelectLaunch::trace).
+struct EngineBufs {
+  double *geo = nullptr;
+  uint8_t *split = nullptr;
+  EvalItem *lists[kNumKinds] = {};
+  uint64_t per_kind[kNumKinds] = {};
+  uint64_t slots = 0;
+  JobDesc *jobs = nullptr;
+  JobState *state = nullptr;
+  TermArgs *args = nullptr;
+  double *boxes = nullptr;
+  uint32_t *dcnt = nullptr;
+  uint32_t *hcnt = nullptr;
+  uint32_t *trace = nullptr;
+  uint32_t trace_cap = 0;
+};
+// Roots and refinement rounds of nj jobs whose descriptors (with their slot
+// bases), parameters and boxes are on the device in B; rounds as one graph
+// launch each (use_graph) or as stream launches.  Fills the timing fields of
+// c->last; the states stay on the device.
+int refine_jobs(g3l_ctx *c, const EngineBufs &B, uint32_t nj, bool timed, int prec,
+                bool use_graph);
+// Evaluation counts and wall time of c->last from the final states.
+void finish_stats(g3l_ctx *c, const JobDesc *desc, const JobState *st, uint32_t nj,
+                  double wall_ms);
 // A context with a stream and timing events but no tables, for engine runs
 // whose integrands need none (the table precomputation).
 int ctx_create_bare(g3l_ctx **out);

@@ -45,6 +45,10 @@ struct SelectLaunch {
   uint32_t list_cap[kNumKinds];
   const uint32_t *active;  // block b selects for job active[b]; null: job b
   uint32_t *done;          // receives each job that finishes this round; may be null
+  // optional batch-size trace: regions cut into by job J's r-th selection
+  // (2 per popped region) at trace[J * trace_cap + r], r < trace_cap; may be null
+  uint32_t *trace;
+  uint32_t trace_cap;
 };
 
 // New integrals joining a running engine: their descriptors and parameters

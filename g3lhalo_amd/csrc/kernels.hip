@@ -16,6 +16,7 @@
 #include "kernels.h"
 #include "physics.h"
 #include "rules.h"
+#include "select_math.h"
 
 #pragma clang fp contract(off)
 
@@ -756,16 +757,6 @@ __device__ __forceinline__ void block_scan2(uint64_t &a, uint64_t &b, uint64_t *
   __syncthreads();
 }
 
-__device__ __forceinline__ uint64_t err_key(double e) {
-  // errmax of src/hcubature.cpp:119-126: max(0, err), NaN -> 0
-  double k = e > 0 ? e : 0.0;
-  return (uint64_t)__double_as_longlong(k);
-}
-
-__device__ __forceinline__ unsigned long long fixed_err(uint64_t k, int shift) {
-  return (unsigned long long)ldexp(__longlong_as_double((long long)k), shift);
-}
-
 __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
   const uint32_t J = L.active ? L.active[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
@@ -875,8 +866,9 @@ __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
   //    tolerance or the evaluation budget is spent (src/hcubature.cpp:971-985).
   //    The pops take the regions in decreasing error; the batch is found by
   //    a radix select over the 64-bit error keys, 8 bits per level from the
-  //    top, in fixed-point error units (maxkey maps below 2^47, so a sum over
-  //    2^16 regions cannot overflow; integer sums are exact and order-free).
+  //    top, in fixed-point error units (sel_shift maps maxkey low enough that
+  //    a sum over all n keys stays below 2^63; integer sums are exact and
+  //    order-free).
   const uint64_t twoP = 2ull * (uint64_t)d.pts;
   uint64_t kcap = n;
   if (d.maxeval != 0) {
@@ -884,8 +876,7 @@ __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
     uint64_t kc = (room + twoP - 1) / twoP;
     kcap = kc < kcap ? kc : kcap;
   }
-  const int ex = (int)((maxkey >> 52) & 0x7ff) - 1023;
-  const int shift = 46 - ex;
+  const int shift = sel_shift(maxkey, n);
   const double thr = ldexp(fabs(V) * d.reltol, shift);
   if (tid == 0) {
     s_prefix = 0;
@@ -923,10 +914,7 @@ __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
     block_scan2(cs, cc, s_wa, s_wb, tot_s, tot_c);
     if (level == 0) {
       // the level-0 histogram holds every key: its sum is the total error
-      uint64_t need;
-      if (thr >= (double)tot_s) need = 1;  // at least one region is popped
-      else need = tot_s - (uint64_t)floor(thr);
-      if (tid == 0) s_need = need == 0 ? 1 : need;
+      if (tid == 0) s_need = batch_need(tot_s, thr);
       __syncthreads();
     }
     const uint64_t need = s_need, cs0 = s_cumsum, cc0 = s_cumcnt;
@@ -1074,6 +1062,7 @@ __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
   }
   if (tid == 0) {
     JobState st = s_st;
+    if (L.trace && st.rounds < L.trace_cap) L.trace[(size_t)J * L.trace_cap + st.rounds] = 2u * s_K;
     st.n = n + s_K;
     st.neval += twoP * s_K;
     st.rounds += 1;

@@ -82,14 +82,20 @@ FP32_PER_EVAL = {
 }
 
 FP64_PEAK = 78.6    # TFLOP/s, MI355X vector FP64
-FP32_PEAK = 157.3   # TFLOP/s, MI355X vector FP32
+FP32_PEAK = 157.3   # TFLOP/s, MI355X vector FP32 (packed / dual-issue instructions)
+# The FP32 rate of the instructions this build emits: packed FP32 is switched
+# off (-packed-fp32-ops), so an FP32 FMA issues one per lane per cycle, as an
+# FP64 FMA does: 256 CUs x 4 SIMDs x 16 lanes x 2 flops x 2.4 GHz.
+FP32_UNPACKED_PEAK = 78.6
 
 
 def fp64_fraction(kind: int) -> float:
     return 1.0 - FP32_PER_EVAL[kind] / FLOPS_PER_EVAL[kind]
 
 
-def mixed_peak_tflops(kind: int) -> float:
-    """1 / (phi64 / FP64 peak + (1 - phi64) / FP32 peak)."""
+def mixed_peak_tflops(kind: int, packed: bool = False) -> float:
+    """1 / (phi64 / FP64 peak + (1 - phi64) / FP32 peak), with the FP32 peak of
+    the instructions emitted (unpacked unless packed=True)."""
     p = fp64_fraction(kind)
-    return 1.0 / (p / FP64_PEAK + (1.0 - p) / FP32_PEAK)
+    f32 = FP32_PEAK if packed else FP32_UNPACKED_PEAK
+    return 1.0 / (p / FP64_PEAK + (1.0 - p) / f32)

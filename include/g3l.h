@@ -148,6 +148,19 @@ typedef struct g3l_batch_stats {
 } g3l_batch_stats;
 int g3l_last_batch_stats(g3l_ctx *ctx, g3l_batch_stats *out);
 
+/* Batch-size trace of g3l_eval_batch (a test and diagnosis hook).  The
+ * reference's integrator evaluates one Gladwell batch of nR regions per
+ * integrand call (src/hcubature.cpp:971-988); its call count and the nR
+ * sequence record which regions were refined.  cap > 0 makes the following
+ * g3l_eval_batch calls record, per integral, the regions of each of its
+ * first cap selection rounds; 0 (the default) stops recording. */
+int g3l_set_trace(g3l_ctx *ctx, uint32_t cap);
+/* Regions evaluated per integrand call by integral i of the last traced
+ * g3l_eval_batch: regions[0] = 1 (the root region), then twice the regions
+ * popped by each selection; *len = min(rounds, trace cap + 1, cap) entries,
+ * where rounds is that integral's g3l_result.rounds. */
+int g3l_last_trace(g3l_ctx *ctx, size_t i, uint32_t *regions, uint32_t cap, uint32_t *len);
+
 /* Block-shape tuning.  No counterpart in the reference (its CUDA build fixes
  * <<<184,256>>>, src/NNMap_Model.cu:596-631).  The evaluation kernels come
  * in launch variants (regions per block) that give identical results;
