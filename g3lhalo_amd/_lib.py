@@ -14,9 +14,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libg3l_hip.so")
 
 G3L_OK, G3L_EINVAL, G3L_ENOMEM, G3L_EHIP, G3L_ENOMEM_POOL, G3L_MAXEVAL_HIT, G3L_NONFINITE, \
-    G3L_ENODEV = range(8)
+    G3L_ENODEV, G3L_ETOOBIG = range(9)
 STATUS_NAMES = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ENOMEM_POOL",
-                5: "MAXEVAL_HIT", 6: "NONFINITE", 7: "ENODEV"}
+                5: "MAXEVAL_HIT", 6: "NONFINITE", 7: "ENODEV", 8: "ETOOBIG"}
+NOALLOC_SCRATCH_BYTES = 64 << 20
 NNMAP, NMAPMAP = 0, 1
 FP64, MIXED = 0, 1
 LAUNCH_STREAM, LAUNCH_GRAPH = 0, 1
@@ -105,6 +106,8 @@ SIGNATURES = {
                                  C.POINTER(Result)]),
     "g3l_integrand": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int32, C.c_size_t, _dp,
                                 _dp]),
+    "g3l_eval_batch_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_size_t,
+                                         C.c_int32, C.POINTER(Result)]),
     "g3l_last_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
     "g3l_set_trace": (C.c_int, [C.c_void_p, C.c_uint32]),
     "g3l_last_trace": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.c_uint32,
@@ -242,6 +245,16 @@ class Context:
         out = (Result * n)()
         check("g3l_eval_batch", load().g3l_eval_batch(self.h, arr, n, int(precision), out))
         return list(out)
+
+    def eval_batch_noalloc(self, reqs, precision=FP64):
+        """g3l_eval_batch_noalloc: the batched engine over the fixed device
+        scratch array (no allocation in the call)."""
+        n = len(reqs)
+        arr = (Request * max(n, 1))(*reqs)
+        out = (Result * max(n, 1))()
+        check("g3l_eval_batch_noalloc",
+              load().g3l_eval_batch_noalloc(self.h, arr, n, int(precision), out))
+        return list(out)[:n]
 
     def set_trace(self, cap: int):
         """Record the batch-size trace of following eval_batch calls (cap

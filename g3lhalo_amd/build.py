@@ -21,7 +21,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument"]
 
-SOURCES = ["kernels.hip", "engine.cpp", "server.cpp", "round_graph.cpp", "precompute.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "server.cpp", "round_graph.cpp", "precompute.cpp",
+           "noalloc.cpp"]
 
 
 def hipcc() -> str:

@@ -149,6 +149,9 @@ int ctx_create_bare(g3l_ctx **out) {
     g3l_ctx_destroy(c);
     return G3L_EHIP;
   }
+  // the allocation-free entry point's array on this device (a failure is
+  // reported here and again by that entry point, which retries)
+  (void)scratch_resolve(c->device);
   *out = c;
   return G3L_OK;
 }

@@ -211,6 +211,10 @@ void finish_stats(g3l_ctx *c, const JobDesc *desc, const JobState *st, uint32_t 
 // whose integrands need none (the table precomputation).
 int ctx_create_bare(g3l_ctx **out);
 
+// Resolves the device address of the allocation-free entry point's scratch
+// array on `device` (the current device); once per device (noalloc.cpp).
+int scratch_resolve(int device);
+
 int kind_of(int stat, int term);
 const double *nbar_ptr(const g3l_ctx *c, int set, int which);
 TermArgs make_args(const g3l_ctx *c, const g3l_request &r);

@@ -30,7 +30,9 @@ typedef enum g3l_status {
   G3L_MAXEVAL_HIT = 5,   /* warning: the evaluation budget ended the integral
                             (the reference returns silently in that case) */
   G3L_NONFINITE = 6,     /* a NaN/Inf reached an integral's totals */
-  G3L_ENODEV = 7         /* no HIP device */
+  G3L_ENODEV = 7,        /* no HIP device */
+  G3L_ETOOBIG = 8        /* the workload exceeds the fixed scratch array of
+                            g3l_eval_batch_noalloc */
 } g3l_status;
 
 typedef enum g3l_stat { G3L_NNMAP = 0, G3L_NMAPMAP = 1 } g3l_stat;
@@ -118,6 +120,25 @@ int g3l_eval_plain(g3l_ctx *ctx, const g3l_request *req, g3l_result *out);
  * above, e.g. NNMap_Model::calculateAll (src/NNMap_Model.cu:382-415). */
 int g3l_eval_batch(g3l_ctx *ctx, const g3l_request *reqs, size_t n, int32_t precision,
                    g3l_result *out);
+
+/* Allocation-free form of g3l_eval_batch.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory during the call --
+ * code under HIP graph capture, serving loops and harnesses that forbid
+ * hipMalloc.  Same arguments, kernels and results as g3l_eval_batch, with
+ * every device buffer of the batched engine (region pool, evaluation lists,
+ * job descriptors and states, integrand parameters, round counters) in one
+ * fixed array of G3L_NOALLOC_SCRATCH_BYTES per device, held until the process
+ * ends.  The call makes no hipMalloc, hipFree or other allocation (the
+ * context and the HOD sets are made beforehand); its refinement rounds are
+ * launched on the context's stream.  A workload whose buffers exceed the
+ * array (about 64 bytes per integral plus 153 bytes per region slot; an
+ * integral at maxeval m holds 2 + (m + pts)/(2 pts) slots, pts = 93, 149,
+ * 241 for terms 1, 2, 3) returns G3L_ETOOBIG.  Every call on one device
+ * shares that device's array: calls that overlap in time on one device race
+ * and must be serialised by the caller. */
+#define G3L_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int g3l_eval_batch_noalloc(g3l_ctx *ctx, const g3l_request *reqs, size_t n, int32_t precision,
+                           g3l_result *out);
 
 /* The integrand of a request at npts points x[npts][dim] (dim 5/6/7 for
  * terms 1/2/3, coordinates log10 l1, log10 l2, phi, log10 m..., z), written
