@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 from g3lhalo_amd import _lib, flops  # noqa: E402
 from g3lhalo_amd import tables as T  # noqa: E402
+from g3lhalo_amd import workloads as W  # noqa: E402
 
 FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X vector FP64
 KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapMap 1h",
@@ -134,41 +135,22 @@ def cpu_baseline(tables, reqs, seconds_budget: float):
             "seconds": round(t_used, 2)}, vals
 
 
-def serve_stream(n: int, seed: int = 4):
-    """A seeded request stream: each request carries an HOD vector from the
-    uniform prior, 1-200 triplets of the 550-triplet grid and a random
-    non-empty mask over NNMap/NMapMap 1h/2h/3h; one request in five is a
-    two-population NNMap request (two prior HODs, types (1,2))."""
-    rng = np.random.default_rng(seed)
-    trips = T.triplets_550()
-    hods = T.hod_prior_draws(2 * n, seed=seed + 1)
-    out = []
-    for i in range(n):
-        two = rng.random() < 0.2
-        k = int(rng.integers(1, 201))
-        sel = trips[rng.choice(len(trips), size=k, replace=False)]
-        while True:
-            mask = int(rng.integers(1, 64))
-            terms = [(0, t) for t in (1, 2, 3) if mask >> (t - 1) & 1]
-            if not two:
-                terms += [(1, t) for t in (1, 2, 3) if mask >> (t + 2) & 1]
-            if terms:
-                break
-        out.append({"hod1": hods[2 * i], "hod2": hods[2 * i + 1] if two else hods[2 * i],
-                    "two": two, "triplets": sel, "terms": terms})
-    return out
-
-
 def serve(ctx, tables, n: int, rate: float, precision=_lib.FP64, seed: int = 4):
     """Continuous batching of a Poisson request stream through g3l_server_*:
     each request's HOD set (device n_bar) is made when it arrives, its
     integrals join the ones in flight at the next refinement round.  Latency
-    runs from the scheduled arrival to the step that completes the request."""
-    stream = serve_stream(n, seed)
+    runs from the scheduled arrival to the end of the step that completes the
+    request, split at the start of the step that admitted its last integral
+    into admission wait (queueing for pool memory and the host loop) and
+    refinement."""
+    stream = W.serve_stream(n, seed)
     rng = np.random.default_rng(seed + 2)
     arrive = np.cumsum(rng.exponential(1.0 / rate, size=n))
     srv = _lib.Server(ctx, pool_slots=1 << 26, max_integrals=1 << 17, precision=precision)
     finish = np.full(n, np.nan)
+    admitted = np.full(n, np.nan)
+    round_start = {}
+    rounds_done = 0
     nint = 0
     i = 0
     t0 = time.perf_counter()
@@ -178,32 +160,41 @@ def serve(ctx, tables, n: int, rate: float, precision=_lib.FP64, seed: int = 4):
             while i < n and arrive[i] <= now:
                 q = stream[i]
                 sid = ctx.hodset(q["hod1"], q["hod2"], identical=not q["two"])
-                reqs = [_lib.make_request(st, t, th * T.ARCMIN_IN_RAD, hodset=sid, type1=1,
-                                          type2=2, reltol=1e-4, maxeval=5000000)
-                        for th in q["triplets"] for (st, t) in q["terms"]]
+                reqs = [_lib.make_request(st, t, th * T.ARCMIN_IN_RAD, hodset=sid, type1=t1,
+                                          type2=t2, reltol=1e-4, maxeval=5000000)
+                        for (st, t, th, t1, t2) in W.request_integrals(q)]
                 nint += len(reqs)
                 srv.submit(reqs, tag=i)
                 i += 1
+            ts = time.perf_counter() - t0
             st = srv.step()
             t = time.perf_counter() - t0
-            for tag, _ in srv.poll():
+            if st.rounds > rounds_done:
+                round_start[rounds_done] = ts
+                rounds_done = st.rounds
+            for tag, _, (_, adm, _) in srv.poll_info():
                 finish[tag] = t
+                admitted[tag] = round_start.get(adm, ts)
             if st.in_flight == 0 and st.pending == 0 and i < n:
                 time.sleep(max(0.0, arrive[i] - (time.perf_counter() - t0)))
     finally:
         srv.close()
     lat = (finish - arrive) * 1e3
+    wait = (admitted - arrive) * 1e3
+    refine = (finish - admitted) * 1e3
     span = float(np.nanmax(finish) - arrive[0])
+
+    def pct(a):
+        return {"p50": round(float(np.percentile(a, 50)), 1),
+                "p99": round(float(np.percentile(a, 99)), 1)}
     return {"value": round(nint / span, 3), "unit": "integrals/s", "requests": n,
             "integrals": nint, "arrival_rate_per_s": rate,
-            "p50_ms": round(float(np.percentile(lat, 50)), 1),
-            "p99_ms": round(float(np.percentile(lat, 99)), 1),
-            "span_s": round(span, 3),
-            "stream": "per request: prior HOD (20% two-population), 1-200 of the 550 "
-                      "triplets, random NNMap/NMapMap 1h/2h/3h mask; tol 1e-4, maxEval 5e6",
-            "note": f"Poisson arrivals at {rate:g}/s: a request averages ~300 integrals, so "
-                    "200/s would be ~20x the device's throughput and only queueing would "
-                    "grow"}
+            "p50_ms": pct(lat)["p50"], "p99_ms": pct(lat)["p99"],
+            "admission_wait_ms": pct(wait), "refinement_ms": pct(refine),
+            "span_s": round(span, 3), "rounds": rounds_done,
+            "stream": "per request: prior HOD (20% two-population with the three type pairs), "
+                      "1-200 of the 550 triplets, random NNMap/NMapMap 1h/2h/3h mask; tol 1e-4, "
+                      "maxEval 5e6"}
 
 
 def vs_fixture(values, name: str, tables):
@@ -221,7 +212,7 @@ def vs_fixture(values, name: str, tables):
                       "frac_above_1e-4": float(np.mean(d[t - 1::3] > 1e-4))} for t in (1, 2, 3)}
 
 
-def hod_sweep(ctx, n_hods: int = 256, pool_slots: int = 1 << 26, precision=_lib.FP64):
+def hod_sweep(ctx, tables, n_hods: int = 256, pool_slots: int = 1 << 26, precision=_lib.FP64):
     """The batched fit sweep: n_hods HOD vectors from the uniform prior x the
     55 triplets (a_i, a_j, a_j) x NNMap 1h/2h/3h, every integral at tol 1e-4
     and maxEval 5e6, submitted at once to one server whose region pool is
@@ -243,17 +234,34 @@ def hod_sweep(ctx, n_hods: int = 256, pool_slots: int = 1 << 26, precision=_lib.
             nint += len(reqs)
             srv.submit(reqs, tag=h)
         rounds = 0
+        first = {}
         while done < n_hods:
             srv.step()
             rounds += 1
-            for _, res in srv.poll():
+            for tag, res in srv.poll():
                 done += 1
                 lat.append(time.perf_counter() - t0)
                 evals += sum(int(r.neval) for r in res)
+                if tag < 8:
+                    first[tag] = [r.value for r in res]
     finally:
         srv.close()
     span = time.perf_counter() - t0
+    parity = None
+    path = os.path.join(ROOT, "tests", "golden", "hod_sweep8_oracle.npz")
+    if os.path.exists(path) and len(first) == 8:
+        g = np.load(path)
+        v = np.concatenate([first[h] for h in range(8)])
+        if str(g["tables_checksum"]) == T.checksum(tables) and len(v) == len(g["value"]):
+            d = np.abs(v - g["value"]) / np.abs(g["value"])
+            parity = {"integrals": int(len(d)), "hod_sets": 8,
+                      "max_rel": {f"{t}h": float(d[t - 1::3].max()) for t in (1, 2, 3)},
+                      "frac_above_1e-4": {f"{t}h": float(np.mean(d[t - 1::3] > 1e-4))
+                                          for t in (1, 2, 3)},
+                      "note": "the first 8 HOD sets of the sweep (all 55 triplets x 3 terms) "
+                              "against the oracle's values (tests/golden/hod_sweep8_oracle.npz)"}
     return {"value": round(nint / span, 3), "unit": "integrals/s", "hod_sets": n_hods,
+            "parity_vs_oracle": parity,
             "integrals": nint, "seconds": round(span, 3), "rounds": rounds,
             "evals": evals, "pool_slots": pool_slots,
             "pool_gib": round(pool_slots * 249 / 2**30, 2),
@@ -261,6 +269,37 @@ def hod_sweep(ctx, n_hods: int = 256, pool_slots: int = 1 << 26, precision=_lib.
             "workload": f"{n_hods} prior HOD vectors x 55 triplets (a_i, a_j, a_j) x NNMap "
                         "1h/2h/3h, tol 1e-4, maxEval 5e6; device n_bar(z) of every HOD set "
                         "inside the timed region; one server, bounded region pool"}
+
+
+def nb256_line():
+    """The nnmap550 workload on 256-bin tables (every table on a 256-point
+    grid, the reference's one bin count for all tables), FP64 batched, one
+    warm-up and one timed step; a sample checked against the oracle's values
+    on the same tables (tests/golden/nnmap550_nb256_oracle.npz)."""
+    t = T.make_tables(nbins=256)
+    c = _lib.Context(t)
+    try:
+        sid = c.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
+        reqs, desc = workload("nnmap550", sid)
+        c.eval_batch(reqs)
+        t0 = time.perf_counter()
+        res = c.eval_batch(reqs)
+        dt = time.perf_counter() - t0
+    finally:
+        c.close()
+    out = {"value": round(len(reqs) / dt, 3), "unit": "integrals/s",
+           "ms_per_step": round(dt * 1e3, 3), "steps": 1, "warmup": 1, **desc, "nbins": 256}
+    path = os.path.join(ROOT, "tests", "golden", "nnmap550_nb256_oracle.npz")
+    if os.path.exists(path):
+        g = np.load(path)
+        if str(g["tables_checksum"]) == T.checksum(t):
+            idx = 3 * g["triplet"] + g["term"] - 1
+            d = np.abs(np.array([res[i].value for i in idx]) - g["value"]) / np.abs(g["value"])
+            out["parity_vs_oracle"] = {
+                "integrals": int(len(d)), "max_rel_1h_2h": float(d[g["term"] < 3].max()),
+                "max_rel_3h": float(d[g["term"] == 3].max()),
+                "sample": "triplets 0, 183, 366, 549 x 1h/2h/3h"}
+    return out
 
 
 def precompute_line(nbins: int = 128, cpu_nbins: int = 32):
@@ -337,14 +376,17 @@ def main(argv=None):
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests in the served run (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=4.0,
-                    help="mean Poisson arrival rate of the served run, requests/s")
+    ap.add_argument("--serve-rates", default="4,8,12",
+                    help="mean Poisson arrival rates of the served runs below capacity, "
+                         "requests/s (comma-separated; the first is the 'served' line)")
     ap.add_argument("--sweep-hods", type=int, default=256,
                     help="HOD vectors in the batched fit sweep (0 skips it)")
     ap.add_argument("--no-nmapmap", action="store_true",
                     help="skip the NMapMap 550-triplet line")
     ap.add_argument("--no-precompute", action="store_true",
                     help="skip the table-precomputation line")
+    ap.add_argument("--no-nb256", action="store_true",
+                    help="skip the 256-bin variant of the workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run an elementwise kernel that reads and writes this many MiB "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -559,17 +601,23 @@ def main(argv=None):
                 "value": round(len(nreqs) / mdt, 3), "unit": "integrals/s",
                 "rel_err_vs_oracle_per_term": vs_fixture([r.value for r in mres], "nmapmap550",
                                                          tables)}
+    if not a.no_nb256 and a.workload == "nnmap550":
+        line["nnmap550_nb256"] = nb256_line()
     if a.sweep_hods > 0:
-        line["hod_sweep"] = hod_sweep(ctx, a.sweep_hods)
+        line["hod_sweep"] = hod_sweep(ctx, tables, a.sweep_hods)
     if not a.no_precompute:
         line["precompute"] = precompute_line()
     if a.serve_requests > 0:
-        line["served"] = serve(ctx, tables, a.serve_requests, a.serve_rate)
+        # below capacity (a request averages ~400 integrals, the device
+        # refines ~4600 integrals/s: ~11 requests/s), then at the serving
+        # configuration's nominal 200 requests/s (overload: latency is
+        # queueing, the throughput is the device's capacity)
+        rates = [float(r) for r in a.serve_rates.split(",") if r]
+        line["served"] = serve(ctx, tables, a.serve_requests, rates[0])
+        line["served_at_rates"] = [line["served"]] + [serve(ctx, tables, a.serve_requests, r)
+                                                      for r in rates[1:]]
         line["latency_ms"]["served_p50"] = line["served"]["p50_ms"]
         line["latency_ms"]["served_p99"] = line["served"]["p99_ms"]
-        # the serving configuration's nominal rate, 200 requests/s: twice as
-        # many requests arrive within a quarter second and queue for admission
-        # (latency is then queueing; the throughput is the device's capacity)
         nominal = serve(ctx, tables, 2 * a.serve_requests, 200.0, seed=7)
         nominal["note"] = ("Poisson arrivals at the nominal 200/s: the stream outruns the device "
                            "and waits for pool memory; throughput at saturation")

@@ -72,6 +72,11 @@ class ServerStatus(C.Structure):
                 ("admitted", C.c_uint32), ("rounds", C.c_uint64)]
 
 
+class ServerRequestInfo(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("submit_round", C.c_uint64), ("admit_round", C.c_uint64),
+                ("done_round", C.c_uint64)]
+
+
 class Cosmology(C.Structure):
     _fields_ = [("Omega_m", C.c_double), ("Omega_Lambda", C.c_double), ("Omega_b", C.c_double),
                 ("sigma8", C.c_double), ("H0", C.c_double), ("ns", C.c_double)]
@@ -114,6 +119,8 @@ SIGNATURES = {
                                  C.POINTER(C.c_uint32)]),
     "g3l_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(TuneReport)]),
     "g3l_set_variant": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    "g3l_set_mixed_policy": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "g3l_get_mixed_policy": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "g3l_set_launch_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "g3l_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "g3l_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_size_t,
@@ -121,6 +128,8 @@ SIGNATURES = {
     "g3l_server_step": (C.c_int, [C.c_void_p, C.POINTER(ServerStatus)]),
     "g3l_server_poll": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_size_t,
                                   C.POINTER(C.c_size_t)]),
+    "g3l_server_poll_info": (C.c_int, [C.c_void_p, C.POINTER(ServerRequestInfo), C.c_size_t,
+                                       C.POINTER(C.c_size_t)]),
     "g3l_server_destroy": (C.c_int, [C.c_void_p]),
     "g3l_precompute": (C.c_int, [C.POINTER(PrecomputeIn), C.POINTER(PrecomputeOut)]),
 }
@@ -290,6 +299,15 @@ class Context:
         check("g3l_set_variant", load().g3l_set_variant(self.h, int(kind), int(precision),
                                                          int(variant)))
 
+    def set_mixed_policy(self, fp32_kinds: int):
+        """Kinds (bit k for kind k) that G3L_MIXED evaluates in FP32 bulk."""
+        check("g3l_set_mixed_policy", load().g3l_set_mixed_policy(self.h, int(fp32_kinds)))
+
+    def mixed_policy(self) -> int:
+        v = C.c_uint32()
+        check("g3l_get_mixed_policy", load().g3l_get_mixed_policy(self.h, C.byref(v)))
+        return v.value
+
     def set_launch_mode(self, mode):
         """LAUNCH_GRAPH (default): one HIP graph launch per refinement round;
         LAUNCH_STREAM: the same kernels launched one by one."""
@@ -345,13 +363,21 @@ class Server:
 
     def poll(self, cap=1024):
         """Completed requests: list of (tag, [Result])."""
-        tags = (C.c_uint64 * cap)()
+        return [(tag, res) for tag, res, _ in self.poll_info(cap)]
+
+    def poll_info(self, cap=1024):
+        """Completed requests: list of (tag, [Result], (submit_round,
+        admit_round, done_round))."""
+        info = (ServerRequestInfo * cap)()
         n = C.c_size_t()
-        check("g3l_server_poll", load().g3l_server_poll(self.h, tags, cap, C.byref(n)))
+        check("g3l_server_poll_info", load().g3l_server_poll_info(self.h, info, cap, C.byref(n)))
         done = []
         for i in range(n.value):
-            arr, out, k = self._out.pop(int(tags[i]))
-            done.append((int(tags[i]), [out[j] for j in range(k)]))
+            tag = int(info[i].tag)
+            arr, out, k = self._out.pop(tag)
+            done.append((tag, [out[j] for j in range(k)],
+                         (int(info[i].submit_round), int(info[i].admit_round),
+                          int(info[i].done_round))))
         return done
 
     def close(self):

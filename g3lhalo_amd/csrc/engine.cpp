@@ -33,6 +33,16 @@ namespace g3l {
 
 int kind_of(int stat, int term) { return stat == G3L_NNMAP ? term : 3 + term; }
 
+KindLaunch kind_launch(const g3l_ctx *c, int prec) {
+  KindLaunch L;
+  for (int k = 0; k < kNumKinds; ++k) {
+    const bool mixed = prec == G3L_MIXED && (c->mixed_fp32_kinds >> k & 1u);
+    L.prec[k] = mixed ? G3L_MIXED : G3L_FP64;
+    L.variant[k] = c->variant[mixed][k];
+  }
+  return L;
+}
+
 const double *nbar_ptr(const g3l_ctx *c, int set, int which) {
   return c->nbar_chunks[set / kSetsPerChunk].as<double>() +
          ((size_t)(set % kSetsPerChunk) * 2 + which) * c->nb;
@@ -218,12 +228,12 @@ int refine_jobs(g3l_ctx *c, const EngineBufs &B, uint32_t nj, bool timed, int pr
   // graph mode: the round's launches and the counter copy are one graph
   // (every node records its events; they are read only when timed)
   RoundGraph &graph = c->graph;
+  const KindLaunch KL = kind_launch(c, prec);
   if (use_graph) {
     // the executable graph is kept while the kinds, variants, precision,
     // job count and buffers stay the same (repeated calls of one workload)
     bool in_run[kNumKinds];
-    const int *var = c->variant[prec == G3L_MIXED];
-    std::vector<uint64_t> key = {(uint64_t)prec, nj, slots, (uint64_t)(uintptr_t)B.geo,
+    std::vector<uint64_t> key = {nj, slots, (uint64_t)(uintptr_t)B.geo,
                                  (uint64_t)(uintptr_t)lists[0], (uint64_t)(uintptr_t)B.jobs,
                                  (uint64_t)(uintptr_t)B.state, (uint64_t)(uintptr_t)B.args,
                                  (uint64_t)(uintptr_t)B.split, (uint64_t)(uintptr_t)dcnt,
@@ -232,11 +242,13 @@ int refine_jobs(g3l_ctx *c, const EngineBufs &B, uint32_t nj, bool timed, int pr
     for (int k = 0; k < kNumKinds; ++k) {
       in_run[k] = per_kind[k] != 0;
       key.push_back(per_kind[k]);
-      key.push_back((uint64_t)var[k]);
+      key.push_back((uint64_t)KL.prec[k]);
+      key.push_back((uint64_t)KL.variant[k]);
     }
     if (!graph.exec || key != c->graph_key) {
       c->graph_key.clear();
-      if ((rc = graph.build(in_run, prec, var, EL, SL, dcnt, kNumKinds + 1, hcnt, c->ev, c->kev)))
+      if ((rc = graph.build(in_run, KL.prec, KL.variant, EL, SL, dcnt, kNumKinds + 1, hcnt, c->ev,
+                            c->kev)))
         return rc;
       c->graph_key = key;
     }
@@ -259,7 +271,7 @@ int refine_jobs(g3l_ctx *c, const EngineBufs &B, uint32_t nj, bool timed, int pr
         EL.list = lists[k];
         EL.count = cnt[k];
         if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k], s));
-        G3L_HIP(launch_eval(k, prec, c->variant[prec == G3L_MIXED][k], EL, s));
+        G3L_HIP(launch_eval(k, KL.prec[k], KL.variant[k], EL, s));
         if (timed) G3L_HIP(hipEventRecord(c->kev[2 * k + 1], s));
       }
       if (timed) G3L_HIP(hipEventRecord(c->ev[1], s));
@@ -998,6 +1010,18 @@ int g3l_autotune(g3l_ctx *c, int32_t hodset, int32_t precision, g3l_tune_report 
 int g3l_set_launch_mode(g3l_ctx *c, int32_t mode) {
   if (!c || (mode != G3L_LAUNCH_STREAM && mode != G3L_LAUNCH_GRAPH)) return G3L_EINVAL;
   c->launch_mode = mode;
+  return G3L_OK;
+}
+
+int g3l_set_mixed_policy(g3l_ctx *c, uint32_t fp32_kinds) {
+  if (!c || (fp32_kinds & ~0x7eu)) return G3L_EINVAL;  // kinds 1-6 only
+  c->mixed_fp32_kinds = fp32_kinds;
+  return G3L_OK;
+}
+
+int g3l_get_mixed_policy(g3l_ctx *c, uint32_t *fp32_kinds) {
+  if (!c || !fp32_kinds) return G3L_EINVAL;
+  *fp32_kinds = c->mixed_fp32_kinds;
   return G3L_OK;
 }
 

@@ -105,7 +105,7 @@ struct RoundGraph {
   void *zero_params[2] = {};
   // ev[0..2]: round start, evaluations done, selection done; kev[2k], kev[2k+1]
   // bracket kind k's evaluation
-  int build(const bool used[kNumKinds], int prec, const int variant[kNumKinds],
+  int build(const bool used[kNumKinds], const int prec[kNumKinds], const int variant[kNumKinds],
             const EvalLaunch &eval_base, const SelectLaunch &sel_base, uint32_t *dcounters,
             int ncounters, uint32_t *hcounters, const hipEvent_t *ev, const hipEvent_t *kev);
   int launch(const uint32_t count[kNumKinds], EvalItem *const lists[kNumKinds], uint32_t nselect,
@@ -145,6 +145,12 @@ struct g3l_ctx {
   // NNMap 1h/2h/3h, NMapMap 1h/2h/3h): g3l_autotune's choices on MI355X for
   // the 550-triplet workloads, 11, 3 and 1 regions per block
   int variant[2][kNumKinds] = {{0, 0, 3, 2, 0, 3, 2}, {0, 0, 3, 2, 0, 3, 2}};
+  // kinds evaluated with the FP32-bulk policy under G3L_MIXED (bit k = kind
+  // k); every other kind runs FP64.  The default keeps FP32 only for the
+  // NNMap 2-halo term, the one kind whose mixed values stayed within 1e-4 of
+  // the FP64 oracle on every integral of the 550-triplet workloads
+  // (g3l_set_mixed_policy).
+  uint32_t mixed_fp32_kinds = 1u << g3l::kNN2;
   // refinement rounds of the batched engine and the server: G3L_LAUNCH_GRAPH
   // (one graph launch per round) or G3L_LAUNCH_STREAM (launches on the stream)
   int launch_mode = G3L_LAUNCH_GRAPH;
@@ -214,6 +220,15 @@ int ctx_create_bare(g3l_ctx **out);
 // Resolves the device address of the allocation-free entry point's scratch
 // array on `device` (the current device); once per device (noalloc.cpp).
 int scratch_resolve(int device);
+
+// Per integrand kind, the precision policy and launch variant of a run at
+// precision prec (G3L_MIXED: PMix for the kinds of mixed_fp32_kinds, P64
+// for the rest).
+struct KindLaunch {
+  int prec[kNumKinds];
+  int variant[kNumKinds];
+};
+KindLaunch kind_launch(const g3l_ctx *c, int prec);
 
 int kind_of(int stat, int term);
 const double *nbar_ptr(const g3l_ctx *c, int set, int which);

@@ -380,7 +380,7 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     const int g = r / 7, c = r - g * 7;
     if (g >= ng || St.slot[g] == 0xffffffffu) return;
     rule_point<D>(kProj<D, bit(2)>.rep[c], St.c[g], St.h[g], x);
-    C.cphi[g][c] = cos(x[2]);
+    C.cphi[g][c] = P::cosphi(x[2]);
   } else if (i < Lay::MZ) {
     const int r = i - Lay::Z;
     if (r >= G * 7) return;

@@ -5,11 +5,13 @@
 //   P64    FP64 throughout, following the reference's expression order
 //          (src/generalFunctions.cu:6-147, src/HOD.cu:35-94,
 //           src/NNMap_Model.cu:544-1108, src/NMapMap_Model.cu:263-660).
-//   PMix   FP32 for the transcendental bulk (the u_NFW core with its Si/Ci,
-//          erf, the exponentials of log-space masses and of the aperture
-//          filters), FP64 for everything that decides a table index or
-//          carries a large dynamic range (coordinates, masses, the nearest-
-//          index chains, table values, products, the NFW normalisation).
+//   PMix   FP32 for the transcendental bulk (the u_NFW core with its Si/Ci
+//          where k r_s is not small, the exponentials of the aperture
+//          filters), FP64 for the small-k r_s branch of u_NFW, the HOD
+//          occupations (erf and the log-space mass exponentials) and
+//          everything that decides a table index or carries a large dynamic
+//          range (coordinates, masses, the nearest-index chains, table
+//          values, products, the NFW normalisation).
 //
 // Deviations of P64 from the reference that change only the last bits:
 // u_NFW by the cancellation-free branch identities noted at P64::u, with
@@ -27,6 +29,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "crmath.h"
 #include "factor.h"
 #include "layout.h"
 
@@ -391,6 +394,7 @@ G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
 struct P64 {
   static constexpr bool kMixed = false;
   G3L_HD static double l(double x) { return exp10(x); }
+  G3L_HD static double cosphi(double x) { return cos(x); }
   // x^(1/3) of r_200
   G3L_HD static double cbrt3(double x) { return cbrt(x); }
 
@@ -470,9 +474,13 @@ struct P64 {
 // The other terms refine identically either way and keep P64's cheaper
 // forms.
 struct P64R : P64 {
-  // 10^x of l1, l2 and x^(1/3) of r_200 as the reference's pow
-  // (src/NNMap_Model.cu:986-991, src/generalFunctions.cu:36)
-  G3L_HD static double l(double x) { return pow(10.0, x); }
+  // 10^x of l1, l2 and cos(phi) correctly rounded (crmath.h), as the
+  // reference's host library returns them for all but ~0.1 % of arguments
+  // (src/NNMap_Model.cu:986-991, 1024): the squeezed bispectrum amplifies a
+  // last-place difference in them; x^(1/3) of r_200 as the reference's pow
+  // (src/generalFunctions.cu:36)
+  G3L_HD static double l(double x) { return cr::exp10_cr(x); }
+  G3L_HD static double cosphi(double x) { return cr::cos_cr(x); }
   G3L_HD static double cbrt3(double x) { return pow(x, 1. / 3.); }
   G3L_HD static Occ occ(const DevHod &h, double lth, double lmp, const Mass &M) {
     (void)lth;
@@ -488,6 +496,22 @@ struct P64R : P64 {
     return pow(M.m, eps);
   }
 };
+
+// u_NFW for a2 < 4 (small k r_s), the FP64 branch of P64::u on its own:
+//   u = 1 + [(cos a1 - 1) ln(1+c) + (a1 c - sin a1 c)/a2
+//            + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si a2 - Si a1)] / F.
+// The Si/Ci differences of nearby arguments and the 1 - O(a1^2) structure
+// need FP64 here; equal to P64::u bit for bit in this branch.
+G3L_HD double u_small64(const Nfw &n, double a1, double a2) {
+  double sx, cx, s1, c1, si1, cr1, si2, cr2;
+  sincos_cw(a1 * n.c, sx, cx);
+  sincos_cw(a1, s1, c1);
+  sici_small_parts(a1, si1, cr1);
+  sici_small_parts(a2, si2, cr2);
+  const double rest =
+      (c1 - 1) * n.lc + fdiv(a1 * n.c - sx, a2) + c1 * (cr2 - cr1) + s1 * (si2 - si1);
+  return 1 + fdiv(rest, n.F);
+}
 
 // ======================================================================
 // Mixed policy: FP32 transcendental bulk.
@@ -578,49 +602,23 @@ G3L_HD void small32(float x, float &si, float &cr) {
   cr = x2 * C / D;
 }
 
-// Scaled complementary error function exp(y^2) erfc(y), y > 0.
-G3L_HD float erfcx32(float y) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return erfcxf(y);
-#else
-  return (float)(std::exp((double)y * y) * std::erfc((double)y));
-#endif
-}
-
-// x - sin(x) without cancellation for |x| < 1 (Taylor to x^11).
-G3L_HD float x_minus_sin(float x, float sx) {
-  if (fabsf(x) < 1.0f) {
-    float x2 = x * x;
-    return x * x2 *
-           fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 2.5052108e-8f, -2.7557319e-6f), 1.9841270e-4f),
-                         -8.3333333e-3f), 0.16666667f);
-  }
-  return x - sx;
-}
-
 }  // namespace mix
 
 struct PMix {
   static constexpr bool kMixed = true;
   G3L_HD static double l(double x) { return exp10(x); }
+  G3L_HD static double cosphi(double x) { return cos(x); }
   G3L_HD static double cbrt3(double x) { return cbrt(x); }
 
-  // log(m/M_th) = L + log(mmin/M_th); 1 + erf(X) = erfc(-X), for X < -1/2
-  // as exp(-X^2) (split precision) times the smooth erfcx(-X) in FP32;
-  // (m/M')^beta = exp(beta (L + log(mmin/M'))).
-  G3L_HD static Occ occ(const DevHod &h, double lth, double lmp,
-                                            const Mass &M) {
-    double X = (M.L + lth) / h.sigma / 1.414213562;
-    double e = X > -0.5 ? (double)erfcf(-(float)X)
-                        : mix::exp_split(-X * X) * (double)mix::erfcx32(-(float)X);
-    Occ o;
-    o.nc = 0.5 * h.alpha * e;
-    o.ns = 0.5 * e * mix::exp_split(h.beta * (M.L + lmp));
-    return o;
+  // The HOD occupations and NsatNsat's power stay FP64 (log-space mass
+  // exponentials and erf, as P64): they are evaluated once per (mass,
+  // redshift) class of a region, so FP32 would save little, and their FP32
+  // rounding moved whole integrals of this precision form by up to 2e-3.
+  G3L_HD static Occ occ(const DevHod &h, double lth, double lmp, const Mass &M) {
+    return P64::occ(h, lth, lmp, M);
   }
-
   G3L_HD static double mpow(const Mass &M, double eps, double lmmin) {
-    return mix::exp_split(eps * (M.L + lmmin));
+    return P64::mpow(M, eps, lmmin);
   }
 
   // Truncated-NFW transform in FP32 with the cancellations removed
@@ -635,6 +633,9 @@ struct PMix {
   G3L_HD static double u(const Nfw &n, double k) {
     double a1d = fdiv(k * n.r200, n.c);
     double a2d = a1d * (1 + n.c);
+    // small k r_s: u = 1 - O(a1^2) from differences of nearby Si/Ci values,
+    // kept in FP64
+    if (a2d < 4.0) return u_small64(n, a1d, a2d);
     double xd = a1d * n.c;
     float sx, cxx;
     mix::sincos_red(xd, sx, cxx);
@@ -650,14 +651,6 @@ struct PMix {
     float a1 = (float)a1d, a2 = (float)a2d;
     float si1, cr1;
     mix::small32(a1, si1, cr1);
-    if (a2d < 4.0) {
-      float si2, cr2;
-      mix::small32(a2, si2, cr2);
-      float h = sinf(0.5f * a1);
-      float rest = -2.0f * h * h * (float)n.lc + mix::x_minus_sin((float)xd, sx) / a2 +
-                   c1 * (cr2 - cr1) + s1 * (si2 - si1);
-      return 1.0 + (double)(rest / (float)n.F);
-    }
     float f2, g2, fm2;
     mix::aux32(a2, f2, g2, fm2);
     const float ci1 = 0.5772156649f + logf(a1) + cr1;
@@ -718,7 +711,7 @@ G3L_HD Geo make_geo(const DevTables &T, const double *x, int D) {
   Geo g;
   g.l1 = P::l(x[0]);
   g.l2 = P::l(x[1]);
-  g.cphi = cos(x[2]);
+  g.cphi = P::cosphi(x[2]);
   g.z = x[D - 1];
   g.zr = zix_round(T, g.z);
   g.zt = zix_trunc(T, g.z);

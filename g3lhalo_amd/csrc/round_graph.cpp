@@ -44,7 +44,8 @@ hipKernelNodeParams kparams(const KernelRef &k, unsigned blocks, void **args) {
 
 }  // namespace
 
-int RoundGraph::build(const bool used[kNumKinds], int prec, const int variant[kNumKinds],
+int RoundGraph::build(const bool used[kNumKinds], const int prec[kNumKinds],
+                      const int variant[kNumKinds],
                       const EvalLaunch &eval_base, const SelectLaunch &sel_base, uint32_t *dcounters,
                       int ncounters, uint32_t *hcounters, const hipEvent_t *ev,
                       const hipEvent_t *kev) {
@@ -56,7 +57,7 @@ int RoundGraph::build(const bool used[kNumKinds], int prec, const int variant[kN
   for (int k = 0; k < kNumKinds; ++k) {
     in_use[k] = used[k];
     if (!used[k]) continue;
-    if (eval_kernel_ref(k, prec, variant[k], &eval_ref[k]) != 0) return G3L_EINVAL;
+    if (eval_kernel_ref(k, prec[k], variant[k], &eval_ref[k]) != 0) return G3L_EINVAL;
     eval_args[k] = eval_base;
     eval_args[k].count = 0;
     eval_params[k][0] = &eval_args[k];

@@ -44,6 +44,7 @@ struct g3l_server {
     uint64_t tag;
     g3l_result *out;
     uint32_t remaining;
+    uint64_t submit_round, admit_round;
   };
   std::vector<Req> reqs;
   std::vector<uint32_t> free_reqs;
@@ -52,13 +53,14 @@ struct g3l_server {
     g3l_request r;
   };
   std::deque<Pending> pending;
-  std::deque<uint64_t> completed;
+  std::deque<g3l_server_request_info> completed;
   uint32_t fill[kNumKinds] = {};  // evaluation-list entries for the next round
   uint64_t rounds = 0;
   // the round as a graph (launch mode G3L_LAUNCH_GRAPH), built at the first
   // step and again whenever the context's launch variants change
   RoundGraph graph;
   int graph_variant[kNumKinds] = {};
+  int graph_prec[kNumKinds] = {};
 };
 
 namespace {
@@ -96,7 +98,9 @@ void finish(g3l_server *s, uint32_t req, uint32_t pos, const g3l_result &r) {
   g3l_server::Req &rq = s->reqs[req];
   rq.out[pos] = r;
   if (--rq.remaining == 0) {
-    s->completed.push_back(rq.tag);
+    // rounds already counts the round that finished it
+    s->completed.push_back(
+        g3l_server_request_info{rq.tag, rq.submit_round, rq.admit_round, s->rounds});
     s->free_reqs.push_back(req);
   }
 }
@@ -132,6 +136,7 @@ int admit(g3l_server *s, uint32_t &admitted) {
     d.base = (uint32_t)base;
     d.args = (int)id;
     s->info[id] = g3l_server::JobInfo{p.req, p.pos, base, d.cap, p.r.stat, p.r.term};
+    s->reqs[p.req].admit_round = s->rounds;
     ids.push_back(id);
     desc.push_back(d);
     args.push_back(make_args(c, p.r));
@@ -254,7 +259,7 @@ int g3l_server_submit(g3l_server *s, const g3l_request *r, size_t n, g3l_result 
       return G3L_EINVAL;  // could never be admitted
   }
   if (n == 0) {
-    s->completed.push_back(tag);
+    s->completed.push_back(g3l_server_request_info{tag, s->rounds, s->rounds, s->rounds});
     return G3L_OK;
   }
   uint32_t q;
@@ -265,7 +270,7 @@ int g3l_server_submit(g3l_server *s, const g3l_request *r, size_t n, g3l_result 
     q = (uint32_t)s->reqs.size();
     s->reqs.push_back({});
   }
-  s->reqs[q] = g3l_server::Req{tag, out, (uint32_t)n};
+  s->reqs[q] = g3l_server::Req{tag, out, (uint32_t)n, s->rounds, s->rounds};
   for (size_t i = 0; i < n; ++i) s->pending.push_back(g3l_server::Pending{q, (uint32_t)i, r[i]});
   return G3L_OK;
 }
@@ -312,15 +317,18 @@ int g3l_server_step(g3l_server *s, g3l_server_status *status) {
   SL.active = s->active.as<uint32_t>();
   SL.done = s->done.as<uint32_t>();
   uint32_t *hc = s->hcounters.as<uint32_t>();
-  const int *var = c->variant[s->precision == G3L_MIXED];
+  const KindLaunch KL = kind_launch(c, s->precision);
+  const int *var = KL.variant;
   if (c->launch_mode == G3L_LAUNCH_GRAPH) {
-    if (!s->graph.exec || !std::equal(var, var + kNumKinds, s->graph_variant)) {
+    if (!s->graph.exec || !std::equal(var, var + kNumKinds, s->graph_variant) ||
+        !std::equal(KL.prec, KL.prec + kNumKinds, s->graph_prec)) {
       bool in_use[kNumKinds];
       for (int k = 0; k < kNumKinds; ++k) in_use[k] = k != kNbar;  // the six terms
-      if ((rc = s->graph.build(in_use, s->precision, var, EL, SL, dcnt, kNumKinds + 2, hc, c->ev,
+      if ((rc = s->graph.build(in_use, KL.prec, var, EL, SL, dcnt, kNumKinds + 2, hc, c->ev,
                                c->kev)))
         return rc;
       std::copy(var, var + kNumKinds, s->graph_variant);
+      std::copy(KL.prec, KL.prec + kNumKinds, s->graph_prec);
     }
     if ((rc = s->graph.launch(s->fill, s->lists_k, na, st))) return rc;
   } else {
@@ -328,7 +336,7 @@ int g3l_server_step(g3l_server *s, g3l_server_status *status) {
       if (!s->fill[k]) continue;
       EL.list = s->lists_k[k];
       EL.count = s->fill[k];
-      G3L_HIP(launch_eval(k, s->precision, var[k], EL, st));
+      G3L_HIP(launch_eval(k, KL.prec[k], var[k], EL, st));
     }
     G3L_HIP(launch_zero(dcnt, kNumKinds + 2, st));
     G3L_HIP(launch_select(SL, na, st));
@@ -377,7 +385,18 @@ int g3l_server_poll(g3l_server *s, uint64_t *tags, size_t cap, size_t *n) {
   if (!s || !n || (cap && !tags)) return G3L_EINVAL;
   size_t k = 0;
   while (k < cap && !s->completed.empty()) {
-    tags[k++] = s->completed.front();
+    tags[k++] = s->completed.front().tag;
+    s->completed.pop_front();
+  }
+  *n = k;
+  return G3L_OK;
+}
+
+int g3l_server_poll_info(g3l_server *s, g3l_server_request_info *info, size_t cap, size_t *n) {
+  if (!s || !n || (cap && !info)) return G3L_EINVAL;
+  size_t k = 0;
+  while (k < cap && !s->completed.empty()) {
+    info[k++] = s->completed.front();
     s->completed.pop_front();
   }
   *n = k;

@@ -66,18 +66,18 @@ def flops_per_eval(stat: int, term: int) -> int:
     return FLOPS_PER_EVAL[kind_of(stat, term)]
 
 
-# FP32 share of the algorithmic work of the mixed-precision integrand:
-# the u_NFW core (all but the three FP64 argument products), the exponential
-# of each aperture filter, erf and the power of each HOD evaluation, the
-# m^eps of NsatNsat.  Everything else (indices, masses, r200, concentrations,
-# table values, products) stays FP64.
+# FP32 share of the algorithmic work of the mixed-precision integrand at
+# most: the u_NFW core (all but the three FP64 argument products; FP32 where
+# k r_s is not small, FP64 below) and the exponential of each aperture
+# filter.  Everything else (indices, masses, r200, concentrations, HOD
+# occupations, table values, products) stays FP64.
 FP32_PER_EVAL = {
-    1: 3 * (UNFW - 3) + 3 * 20 + 2 * 40 + 20,
-    2: 6 * (UNFW - 3) + 3 * 20 + 4 * 40 + 20,
-    3: 3 * (UNFW - 3) + 3 * 20 + 2 * 40,
-    4: 3 * (UNFW - 3) + 3 * 20 + 1 * 40,
-    5: 6 * (UNFW - 3) + 3 * 20 + 2 * 40,
-    6: 3 * (UNFW - 3) + 3 * 20 + 1 * 40,
+    1: 3 * (UNFW - 3) + 3 * 20,
+    2: 6 * (UNFW - 3) + 3 * 20,
+    3: 3 * (UNFW - 3) + 3 * 20,
+    4: 3 * (UNFW - 3) + 3 * 20,
+    5: 6 * (UNFW - 3) + 3 * 20,
+    6: 3 * (UNFW - 3) + 3 * 20,
     0: 0,
 }
 

@@ -37,7 +37,11 @@ typedef enum g3l_status {
 
 typedef enum g3l_stat { G3L_NNMAP = 0, G3L_NMAPMAP = 1 } g3l_stat;
 
-/* Precision of the integrand evaluation in g3l_eval_batch. */
+/* Precision of the integrand evaluation in g3l_eval_batch.  G3L_FP64: every
+ * integrand in FP64.  G3L_MIXED: the kinds selected by g3l_set_mixed_policy
+ * with the FP32-bulk integrand (FP32 u_NFW where k r_s is not small and FP32
+ * aperture exponentials; FP64 small-k r_s u_NFW, HOD occupations, indices,
+ * table values, products and all region sums), the others in FP64. */
 typedef enum g3l_precision { G3L_FP64 = 0, G3L_MIXED = 1 } g3l_precision;
 
 typedef struct g3l_ctx g3l_ctx;
@@ -195,6 +199,16 @@ typedef struct g3l_tune_report {
   int32_t group[8][4];
 } g3l_tune_report;
 int g3l_autotune(g3l_ctx *ctx, int32_t hodset, int32_t precision, g3l_tune_report *out);
+/* Which integrand kinds G3L_MIXED evaluates with the FP32-bulk integrand:
+ * bit k for kind k (1-3 NNMap 1h-3h, 4-6 NMapMap 1h-3h); the others run FP64.
+ * No counterpart in the reference (FP64 throughout).  The default is the
+ * NNMap 2-halo term only (bit 2): integrals that stop at the evaluation cap
+ * unconverged change with any FP32 rounding in their point values (near-tied
+ * regions are split in another order), and on the 550-triplet workloads only
+ * that kind's FP32 values stayed within 1e-4 of the FP64 oracle for every
+ * integral. */
+int g3l_set_mixed_policy(g3l_ctx *ctx, uint32_t fp32_kinds);
+int g3l_get_mixed_policy(g3l_ctx *ctx, uint32_t *fp32_kinds);
 /* Sets one kind's variant directly (kind 1-6, variant 0-3). */
 int g3l_set_variant(g3l_ctx *ctx, int32_t kind, int32_t precision, int32_t variant);
 
@@ -282,6 +296,18 @@ int g3l_server_submit(g3l_server *srv, const g3l_request *reqs, size_t n, g3l_re
 int g3l_server_step(g3l_server *srv, g3l_server_status *status);
 /* Tags of completed requests, in completion order (at most cap). */
 int g3l_server_poll(g3l_server *srv, uint64_t *tags, size_t cap, size_t *n);
+/* The same, with each request's round stamps: submit_round = rounds run
+ * before it was submitted, admit_round = rounds run before the round that
+ * admitted its last integral, done_round = rounds run when it completed
+ * (g3l_server_status.rounds counts).  A client that notes the time of each
+ * step splits the request's latency into admission wait and refinement. */
+typedef struct g3l_server_request_info {
+  uint64_t tag;
+  uint64_t submit_round;
+  uint64_t admit_round;
+  uint64_t done_round;
+} g3l_server_request_info;
+int g3l_server_poll_info(g3l_server *srv, g3l_server_request_info *info, size_t cap, size_t *n);
 int g3l_server_destroy(g3l_server *srv);
 
 #ifdef __cplusplus

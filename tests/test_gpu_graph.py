@@ -36,6 +36,7 @@ def test_graph_rounds_equal_stream_rounds(tables, precision):
     say(f"graph vs stream rounds, precision {precision}")
     c = _lib.Context(tables)
     try:
+        c.set_mixed_policy(0x7e)   # every term in the FP32-bulk form under MIXED
         hods = T.hod_prior_draws(4, seed=31)
         c.set_launch_mode(_lib.LAUNCH_STREAM)
         first_s = c.hodsets(hods, hods)

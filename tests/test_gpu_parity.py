@@ -157,7 +157,12 @@ def test_mixed_batch_terms(ctx, oracle_model, theta):
     th = np.array(theta) * ARCMIN
     reqs = [_lib.make_request(s, t, th, reltol=1e-4, maxeval=1000000) for s, t in TERMS]
     say(f"mixed batch six terms theta={theta}")
-    out = ctx.eval_batch(reqs, precision=_lib.MIXED)
+    keep = ctx.mixed_policy()
+    ctx.set_mixed_policy(0x7e)   # the FP32-bulk form for every term
+    try:
+        out = ctx.eval_batch(reqs, precision=_lib.MIXED)
+    finally:
+        ctx.set_mixed_policy(keep)
     for (s, t), r in zip(TERMS, out):
         o = oracle_model.eval_term(s, t, th, 1, 2, 1e-4, 1000000)
         d = rel(r.value, o["value"])

@@ -48,6 +48,7 @@ def test_server_results_equal_batch(sctx, precision):
     c, a, b = sctx
     groups = _requests(a, b)
     say(f"server precision={precision}: staggered submissions")
+    c.set_mixed_policy(0x7e)   # every term in the FP32-bulk form under MIXED
     srv = _lib.Server(c, pool_slots=1 << 22, max_integrals=4096, precision=precision)
     try:
         srv.submit(groups[0], tag=1)

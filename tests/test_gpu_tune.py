@@ -16,6 +16,7 @@ def test_variants_give_identical_results(tables, precision):
     from g3lhalo_amd import _lib
     c = _lib.Context(tables)
     try:
+        c.set_mixed_policy(0x7e)   # every term in the FP32-bulk form under MIXED
         sid = c.hodset(T.FIDUCIAL_HOD, tuple(T.hod_prior_draws(1, seed=2)[0]))
         reqs = [_lib.make_request(s, t, np.array(th) * ARCMIN, hodset=sid, reltol=1e-4,
                                   maxeval=400000)
