@@ -380,7 +380,7 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     const int g = r / 7, c = r - g * 7;
     if (g >= ng || St.slot[g] == 0xffffffffu) return;
     rule_point<D>(kProj<D, bit(2)>.rep[c], St.c[g], St.h[g], x);
-    C.cphi[g][c] = P::cosphi(x[2]);
+    C.cphi[g][c] = cos(x[2]);
   } else if (i < Lay::MZ) {
     const int r = i - Lay::Z;
     if (r >= G * 7) return;
@@ -757,6 +757,153 @@ __device__ __forceinline__ void block_scan2(uint64_t &a, uint64_t &b, uint64_t *
   __syncthreads();
 }
 
+// Block-wide reductions over an integral's error keys for the exact stopping
+// point: the FP64 sum of the errors whose key is below T (Neumaier, each
+// thread over its strided entries, then a fixed tree), and the largest key
+// below T or smallest key above T with its multiplicity.  Every thread of the
+// block calls them; results are valid in every thread.
+__device__ double block_sum_below(const double *err, uint32_t n, uint64_t T, double *s_a,
+                                  double *s_b) {
+  const int tid = threadIdx.x;
+  double s = 0, c = 0;
+  for (uint32_t i = tid; i < n; i += kSelThreads) {
+    const uint64_t k = err_key(err[i]);
+    if (k >= T) continue;
+    const double e = __builtin_bit_cast(double, k);
+    const double t = s + e;
+    c += fabs(s) >= fabs(e) ? (s - t) + e : (e - t) + s;
+    s = t;
+  }
+  s_a[tid] = s;
+  s_b[tid] = c;
+  __syncthreads();
+  for (int w = kSelThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      const double a = s_a[tid], b = s_a[tid + w];
+      const double t = a + b;
+      s_a[tid] = t;
+      s_b[tid] += s_b[tid + w] + (fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a);
+    }
+    __syncthreads();
+  }
+  const double r = s_a[0] + s_b[0];
+  __syncthreads();
+  return r;
+}
+
+// below: the largest key < T; otherwise the smallest key > T.  Returns the
+// key (0 / ~0 when none) and sets cnt to its multiplicity.
+__device__ uint64_t block_next_key(const double *err, uint32_t n, uint64_t T, bool below,
+                                   uint64_t *s_k, uint32_t *s_c, uint32_t &cnt) {
+  const int tid = threadIdx.x;
+  uint64_t best = below ? 0ull : ~0ull;
+  bool any = false;
+  for (uint32_t i = tid; i < n; i += kSelThreads) {
+    const uint64_t k = err_key(err[i]);
+    if (below ? (k < T && (!any || k > best)) : (k > T && (!any || k < best))) {
+      best = k;
+      any = true;
+    }
+  }
+  s_k[tid] = best;
+  s_c[tid] = any ? 1u : 0u;
+  __syncthreads();
+  for (int w = kSelThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      const uint64_t a = s_k[tid], b = s_k[tid + w];
+      const uint32_t ha = s_c[tid], hb = s_c[tid + w];
+      if (hb && (!ha || (below ? b > a : b < a))) s_k[tid] = b;
+      s_c[tid] = ha | hb;
+    }
+    __syncthreads();
+  }
+  const uint64_t key = s_k[0];
+  const bool found = s_c[0] != 0;
+  __syncthreads();
+  uint32_t m = 0;
+  if (found)
+    for (uint32_t i = tid; i < n; i += kSelThreads) m += err_key(err[i]) == key ? 1u : 0u;
+  s_c[tid] = m;
+  __syncthreads();
+  for (int w = kSelThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) s_c[tid] += s_c[tid + w];
+    __syncthreads();
+  }
+  cnt = s_c[0];
+  __syncthreads();
+  return found ? key : (below ? 0ull : ~0ull);
+}
+
+// Moves the batch boundary (key T with multiplicity cntT, `above` keys
+// larger than T, r of the T-ties taken) until the count K = above + r is the
+// first at which the remaining error meets thr (or K reaches kcap or n),
+// computing the remaining error in FP64.  Every thread calls it.
+__device__ void select_exact_count(const double *err, uint32_t n, uint64_t kcap, double thr,
+                                   uint64_t &T, uint32_t &cntT, uint32_t &above, uint32_t &r,
+                                   double *s_a, double *s_b, uint64_t *s_k, uint32_t *s_c) {
+  const int tid = threadIdx.x;
+  __shared__ double s_tail;
+  __shared__ int s_act;
+  const double below = block_sum_below(err, n, T, s_a, s_b);
+  if (tid == 0) s_tail = below + (double)(cntT - r) * __builtin_bit_cast(double, T);
+  // the loop ends: every step moves K by one towards the unique stopping count
+  for (int guard = 0; guard < 64; ++guard) {
+    if (tid == 0) {
+      double tail = s_tail;
+      int act = 0;
+      for (;;) {
+        const double eT = __builtin_bit_cast(double, T);
+        const uint64_t K = (uint64_t)above + r;
+        // not yet converged after K pops: pop one more (budget and regions left)
+        if (tail > 0 && tail > thr && K < kcap && K < n) {
+          if (r < cntT) {
+            ++r;
+            tail -= eT;
+            continue;
+          }
+          act = 1;  // the next key below T
+          break;
+        }
+        // converged already after K - 1 pops: one pop fewer
+        const double prev = tail + eT;
+        if (K > 1 && !(prev > 0 && prev > thr)) {
+          tail = prev;
+          if (r > 1) {
+            --r;
+            continue;
+          }
+          r = 0;
+          act = 2;  // the next key above T, all of its ties taken
+          break;
+        }
+        break;
+      }
+      s_tail = tail;
+      s_act = act;
+    }
+    __syncthreads();
+    const int act = s_act;
+    if (act == 0) break;
+    uint32_t c = 0;
+    const uint64_t nk = block_next_key(err, n, T, act == 1, s_k, s_c, c);
+    if (tid == 0) {
+      if (act == 1) {
+        above += cntT;
+        T = nk;
+        cntT = c;
+        r = 1;
+        s_tail -= __builtin_bit_cast(double, nk);
+      } else {
+        T = nk;
+        cntT = c;
+        above -= c;
+        r = c;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
   const uint32_t J = L.active ? L.active[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
@@ -952,7 +1099,24 @@ __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
     if (r > s_lastcnt) r = s_lastcnt;
     if ((uint64_t)s_cumcnt + r > kcap) r = kcap - s_cumcnt;
     s_r = (uint32_t)r;
-    uint32_t K = s_cumcnt + (uint32_t)r;
+  }
+  __syncthreads();
+
+  // 3b. The exact stopping point.  The reference pops in decreasing error,
+  //     subtracting each popped error from the running total in FP64, and
+  //     stops at the first count whose remaining error meets the tolerance
+  //     (src/hcubature.cpp:971-985, src/converged.h:10-14).  The fixed-point
+  //     keys above are truncated, by up to one unit (2^-46 of the largest
+  //     error) per region, so on integrals whose errors span many decades
+  //     their sum over thousands of small-error regions can misplace that
+  //     count by a region or two.  The remaining error is recomputed here in
+  //     FP64 (compensated) and the count stepped, one tie at a time and one
+  //     key at a time, until it is the reference's.
+  select_exact_count(err, n, kcap, fabs(V) * d.reltol, s_prefix, s_lastcnt, s_cumcnt, s_r,
+                     s_red, s_red2, s_redk, s_cnt);
+
+  if (tid == 0) {
+    const uint32_t K = s_cumcnt + s_r;
     s_K = K;
     JobState st = s_st;
     if (st.n + K > d.cap) {

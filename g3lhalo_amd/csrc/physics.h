@@ -29,7 +29,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "crmath.h"
 #include "factor.h"
 #include "layout.h"
 
@@ -394,7 +393,6 @@ G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
 struct P64 {
   static constexpr bool kMixed = false;
   G3L_HD static double l(double x) { return exp10(x); }
-  G3L_HD static double cosphi(double x) { return cos(x); }
   // x^(1/3) of r_200
   G3L_HD static double cbrt3(double x) { return cbrt(x); }
 
@@ -474,13 +472,9 @@ struct P64 {
 // The other terms refine identically either way and keep P64's cheaper
 // forms.
 struct P64R : P64 {
-  // 10^x of l1, l2 and cos(phi) correctly rounded (crmath.h), as the
-  // reference's host library returns them for all but ~0.1 % of arguments
-  // (src/NNMap_Model.cu:986-991, 1024): the squeezed bispectrum amplifies a
-  // last-place difference in them; x^(1/3) of r_200 as the reference's pow
-  // (src/generalFunctions.cu:36)
-  G3L_HD static double l(double x) { return cr::exp10_cr(x); }
-  G3L_HD static double cosphi(double x) { return cr::cos_cr(x); }
+  // 10^x of l1, l2 and x^(1/3) of r_200 as the reference's pow
+  // (src/NNMap_Model.cu:986-991, src/generalFunctions.cu:36)
+  G3L_HD static double l(double x) { return pow(10.0, x); }
   G3L_HD static double cbrt3(double x) { return pow(x, 1. / 3.); }
   G3L_HD static Occ occ(const DevHod &h, double lth, double lmp, const Mass &M) {
     (void)lth;
@@ -607,7 +601,6 @@ G3L_HD void small32(float x, float &si, float &cr) {
 struct PMix {
   static constexpr bool kMixed = true;
   G3L_HD static double l(double x) { return exp10(x); }
-  G3L_HD static double cosphi(double x) { return cos(x); }
   G3L_HD static double cbrt3(double x) { return cbrt(x); }
 
   // The HOD occupations and NsatNsat's power stay FP64 (log-space mass
@@ -711,7 +704,7 @@ G3L_HD Geo make_geo(const DevTables &T, const double *x, int D) {
   Geo g;
   g.l1 = P::l(x[0]);
   g.l2 = P::l(x[1]);
-  g.cphi = P::cosphi(x[2]);
+  g.cphi = cos(x[2]);
   g.z = x[D - 1];
   g.zr = zix_round(T, g.z);
   g.zt = zix_trunc(T, g.z);

@@ -1,18 +1,36 @@
 """Full-size parity: the 1650-integral NNMap and NMapMap workloads (550
 aperture triplets x 1h/2h/3h, tol 1e-4, maxEval 5e6) on the device against the
-oracle's values for every integral (tests/golden/nnmap550_oracle.npz,
-tests/golden/nmapmap550_oracle.npz).
+oracle's values, integrand-call counts and batch-size traces for every
+integral (tests/golden/nnmap550_oracle.npz, tests/golden/nmapmap550_oracle.npz).
 
-Every integral of this workload stops at the evaluation cap unconverged; its
-value depends on which regions were split, and a last-place difference in an
-integrand value can change that choice.  Every NMapMap integral and the
-NNMap 1-halo and 2-halo integrals refine identically (equal evaluation counts,
-values equal to rounding); the 7-dimensional NNMap 3-halo integrals, whose
-error estimates are 3 % to several times their values, mostly do.  Where they
-do not, the difference is of the size the oracle itself shows when its math
-library moves by one unit in the last place (tests/golden/
-nnmap550_oracle_ulp.npz: the same few integrals move, by up to ~5e-4), and it
-stays far inside the integral's own error estimate."""
+Every integral of this workload stops at the evaluation cap unconverged, so
+its evaluation count is fixed by arithmetic whatever was refined; the number
+of integrand calls (device: g3l_result.rounds) and the regions evaluated per
+call (device: g3l_last_trace) record the refinement sequence.  Every NMapMap
+integral and every NNMap 1-halo and 2-halo integral refines identically: the
+same calls, the same trace, values equal to rounding.
+
+The 7-dimensional NNMap 3-halo integrals take the oracle's call count
+everywhere and mostly its trace, but their refinement is chaotic at the
+last place: their error estimates are 3 % to 70x their values, their
+squeezed bispectrum configurations (k3 -> 0) amplify rounding, and a
+last-place change in any integrand factor can change which of two near-tied
+regions is split, after which the two runs drift apart by up to the
+integral's own error.  The oracle itself, with every exp, log, pow, sin, cos
+and erf moved by at most one unit in the last place (four seeds,
+tests/golden/nnmap550_oracle_ulp.npz), moves 3 to 8 of the 550 by more than
+1e-4 per seed (13 distinct integrals, up to 2.5e-3): no implementation other
+than the reference binary with its own math library can agree with it to
+1e-4 on every one.  The bar for this term is therefore, per integral: within
+1e-4 of the oracle, or one of the integrals whose value the oracle itself
+does not reproduce to 1e-4 under a one-ulp change of its math library; and
+within 10 % of the oracle's error estimate.  In aggregate: at most as many
+integrals beyond 1e-4, and by at most as much, as one of the oracle's own
+one-ulp runs; the median at the rounding level.
+
+The mixed form (G3L_MIXED with the default per-kind policy) is held to the
+same bars: every 1-halo and 2-halo integral and every NMapMap integral
+within 1e-4 of the FP64 oracle."""
 import os
 
 import numpy as np
@@ -25,47 +43,115 @@ pytestmark = pytest.mark.gpu
 GOLD = {0: os.path.join(ROOT, "tests", "golden", "nnmap550_oracle.npz"),
         1: os.path.join(ROOT, "tests", "golden", "nmapmap550_oracle.npz")}
 ULP = os.path.join(ROOT, "tests", "golden", "nnmap550_oracle_ulp.npz")
+PTS = np.array([93, 149, 241] * 550)
+
+
+def _run(tables, stat, precision=0, trace=True):
+    from g3lhalo_amd import _lib
+    c = _lib.Context(tables)
+    try:
+        sid = c.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
+        if trace:
+            c.set_trace(64)
+        reqs = [_lib.make_request(stat, term, th * T.ARCMIN_IN_RAD, hodset=sid, reltol=1e-4,
+                                  maxeval=5000000)
+                for th in T.triplets_550() for term in (1, 2, 3)]
+        say(f"full 550-triplet batch, stat {stat}, precision {precision}")
+        out = c.eval_batch(reqs, precision)
+        traces = [c.last_trace(i) for i in range(len(reqs))] if trace else None
+        policy = c.mixed_policy()
+    finally:
+        c.close()
+    return out, traces, policy
 
 
 @pytest.mark.parametrize("stat", [0, 1])
 def test_550_against_oracle(tables, stat):
-    from g3lhalo_amd import _lib
     g = np.load(GOLD[stat])
     assert str(g["tables_checksum"]) == T.checksum(tables)
-    c = _lib.Context(tables)
-    try:
-        sid = c.hodset(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
-        reqs = [_lib.make_request(stat, term, th * T.ARCMIN_IN_RAD, hodset=sid, reltol=1e-4,
-                                  maxeval=5000000)
-                for th in T.triplets_550() for term in (1, 2, 3)]
-        say(f"full 550-triplet batch, stat {stat}")
-        out = c.eval_batch(reqs)
-    finally:
-        c.close()
+    out, traces, _ = _run(tables, stat)
     v = np.array([o.value for o in out])
     ne = np.array([o.neval for o in out])
+    rounds = np.array([o.rounds for o in out])
     d = np.abs(v - g["value"]) / np.abs(g["value"])
     err_value = np.abs(g["err"] * g["value"] / g["raw"])   # the oracle's error in value units
+    has_seq = "ncalls" in g.files   # fixtures written before call counts were recorded lack it
+    if has_seq:
+        otrace = g["trace"] // PTS[:, None]
+        same_trace = np.array([np.array_equal(tr, otrace[i, :g["ncalls"][i]])
+                               for i, tr in enumerate(traces)])
+    else:
+        print("fixture without integrand-call counts: refinement sequence not compared")
+        same_trace = np.ones(len(out), dtype=bool)
     for term in (1, 2, 3):
         dt = d[term - 1::3]
         print(f"term {term}: median {np.median(dt):.2e} p99 {np.quantile(dt, 0.99):.2e} "
-              f"max {dt.max():.2e} above 1e-4: {(dt > 1e-4).sum()}")
+              f"max {dt.max():.2e} above 1e-4: {(dt > 1e-4).sum()}; traces equal "
+              f"{same_trace[term - 1::3].sum()} / 550")
     assert np.array_equal(ne, g["neval"])
+    # the refinement sequence: the oracle's call count for every integral
+    assert not has_seq or np.array_equal(rounds, g["ncalls"])
+    # and its batch-size trace for every 1-halo and 2-halo integral
+    assert same_trace[0::3].all() and same_trace[1::3].all()
     assert d[0::3].max() < 1e-10 and d[1::3].max() < 1e-10
     if stat == 1:
         # NMapMap's 3-halo integrals refine identically too
+        assert same_trace[2::3].all()
         assert d[2::3].max() < 1e-10
         return
-    d3 = d[2::3]
-    # the scale set by the reference's algorithm itself: the oracle's 3-halo
-    # values under one-ulp perturbations of its math library (two seeds)
+    check_3h(v, g, err_value)
+
+
+def check_3h(v, g, err_value):
+    """The NNMap 3-halo bar (module docstring)."""
+    d = np.abs(v - g["value"]) / np.abs(g["value"])
+    d3, v3, o3 = d[2::3], v[2::3], g["value"][2::3]
     u = np.load(ULP)
-    assert str(u["tables_checksum"]) == T.checksum(tables)
-    du = np.abs(u["value"] - g["value"][2::3]) / np.abs(g["value"][2::3])
-    n_ulp = int((du > 1e-4).sum(axis=1).max())
-    print(f"NNMap 3h: device above 1e-4: {(d3 > 1e-4).sum()}, max {d3.max():.2e}; oracle under "
-          f"1-ulp perturbation: above 1e-4 at most {n_ulp}, max {du.max():.2e}")
-    assert (d3 > 1e-4).sum() <= 3 * max(n_ulp, 1)
-    assert d3.max() <= 2 * du.max()
-    assert np.median(d3) < 1e-7
+    assert str(u["tables_checksum"]) == str(g["tables_checksum"])
+    assert len(u["seeds"]) >= 4
+    du = np.abs(u["value"] - o3) / np.abs(o3)
+    n_seed, max_seed = int((du > 1e-4).sum(axis=1).max()), float(du.max())
+    lo = np.minimum(o3, u["value"].min(axis=0))
+    hi = np.maximum(o3, u["value"].max(axis=0))
+    out = np.flatnonzero(d3 > 1e-4)
+    inside = [(lo[i] <= v3[i] <= hi[i]) for i in out]
+    sensitive = (du > 1e-4).any(axis=0)
+    print(f"NNMap 3h: {len(out)} above 1e-4 (max {d3.max():.2e}); the oracle's one-ulp runs: at "
+          f"most {n_seed} per seed, {int(sensitive.sum())} distinct, max {max_seed:.2e}; of the "
+          f"device's, {sum(inside)} inside the oracle's one-ulp range, "
+          f"{int(sensitive[out].sum())} among the oracle's own one-ulp movers")
+    for i in out:
+        print(f"  triplet {i}: rel {d3[i]:.2e}, err estimate / |value| "
+              f"{err_value[3 * i + 2] / abs(o3[i]):.2e}")
     assert np.all(np.abs(v - g["value"]) < 0.1 * err_value)
+    # per integral: beyond 1e-4 only where the oracle itself, under a one-ulp
+    # change of its math library, does not reproduce its value to 1e-4
+    assert sensitive[out].all(), out[~sensitive[out]]
+    assert len(out) <= n_seed and d3.max() <= max_seed
+    assert np.median(d3) < 1e-7
+
+
+@pytest.mark.parametrize("stat", [0, 1])
+def test_550_mixed_policy_within_bar(tables, stat):
+    """The mixed form under its default per-kind policy: every integral of
+    the workload within 1e-4 of the FP64 oracle."""
+    g = np.load(GOLD[stat])
+    assert str(g["tables_checksum"]) == T.checksum(tables)
+    out, _, policy = _run(tables, stat, precision=1, trace=False)
+    assert policy != 0
+    v = np.array([o.value for o in out])
+    d = np.abs(v - g["value"]) / np.abs(g["value"])
+    for term in (1, 2, 3):
+        kind = term + 3 * stat
+        dt = d[term - 1::3]
+        print(f"mixed stat {stat} term {term} ({'FP32 bulk' if policy >> kind & 1 else 'FP64'}): "
+              f"max {dt.max():.2e}, above 1e-4: {(dt > 1e-4).sum()}")
+    assert d[0::3].max() <= 1e-4 and d[1::3].max() <= 1e-4
+    if stat == 1:
+        assert d[2::3].max() <= 1e-4
+    elif policy >> 3 & 1:
+        assert d[2::3].max() <= 1e-4
+    else:
+        # the 3-halo term runs FP64 under this policy: the FP64 bar
+        err_value = np.abs(g["err"] * g["value"] / g["raw"])
+        check_3h(v, g, err_value)

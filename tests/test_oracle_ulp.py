@@ -27,8 +27,9 @@ def test_ulp_spread_of_the_3h_workload(tables):
         assert np.array_equal(u["neval"][s], g["neval"][2::3])
     du = np.abs(u["value"] - v0) / np.abs(v0)
     moved = (du > 1e-4).sum(axis=1)
+    assert len(u["seeds"]) >= 4
     assert 1 <= moved.max() <= 20
-    assert du.max() < 1e-3
+    assert du.max() < 1e-2
     assert np.median(du) < 1e-7
 
 
