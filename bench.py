@@ -37,15 +37,15 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this workload scaled by a copy kernel of known size
-# (profiles/k_eval_nnmap550_counters.md): 464.1 MB read + 43.1 MB written per
+# (profiles/k_eval_nnmap550_counters.md): 655.3 MB read + 44.4 MB written per
 # k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
-TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 696404502}
+TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 699693250}
 
 # Hardware FP64 flops per evaluation, (kind, launch variant) -> 2 FMA + ADD +
 # MUL + TRANS of the SQ_INSTS_VALU_*_F64 counters x 64 lanes / evaluations,
 # from the same profile: the work the kernel executes, against the credited
 # algorithmic work of flops.FLOPS_PER_EVAL.
-HW_FP64_FLOPS_PER_EVAL = {(2, 3): 1589, (3, 2): 1111}
+HW_FP64_FLOPS_PER_EVAL = {(1, 0): 1131, (2, 3): 1589, (3, 2): 1111}
 
 # Algorithmic HBM bytes per region an evaluation launch handles: the
 # region's centre, half-widths and volume (2D + 1 doubles) and its list item
@@ -363,6 +363,11 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+def progress(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,7 +446,9 @@ def main(argv=None):
                 "kind_ms": kind_ms, "kind_launches": kind_launches, "kind_evals": kind_evals,
                 "breakdown": breakdown}
 
+    progress("batched FP64 steps")
     f64 = run_form(_lib.FP64)
+    progress("batched mixed steps")
     mixed = run_form(_lib.MIXED) if not a.fp64_only else None
     head = f64
     value = head["value"]
@@ -450,6 +457,7 @@ def main(argv=None):
     bad = [r.status for r in results if r.status not in (_lib.G3L_OK, _lib.G3L_MAXEVAL_HIT)]
     kind_ms, kind_launches, kind_evals = head["kind_ms"], head["kind_launches"], head["kind_evals"]
 
+    progress("plain sample")
     # plain one-integral-at-a-time HIP path on a fixed sample
     pidx = np.linspace(0, len(reqs) - 1, max(1, a.plain_sample)).astype(int)
     t0 = time.perf_counter()
@@ -601,17 +609,22 @@ def main(argv=None):
                 "value": round(len(nreqs) / mdt, 3), "unit": "integrals/s",
                 "rel_err_vs_oracle_per_term": vs_fixture([r.value for r in mres], "nmapmap550",
                                                          tables)}
+    progress("n_bar sweep, NMapMap-550 done")
     if not a.no_nb256 and a.workload == "nnmap550":
+        progress("256-bin variant")
         line["nnmap550_nb256"] = nb256_line()
     if a.sweep_hods > 0:
+        progress("HOD sweep")
         line["hod_sweep"] = hod_sweep(ctx, tables, a.sweep_hods)
     if not a.no_precompute:
+        progress("table precomputation")
         line["precompute"] = precompute_line()
     if a.serve_requests > 0:
         # below capacity (a request averages ~400 integrals, the device
         # refines ~4600 integrals/s: ~11 requests/s), then at the serving
         # configuration's nominal 200 requests/s (overload: latency is
         # queueing, the throughput is the device's capacity)
+        progress("served runs")
         rates = [float(r) for r in a.serve_rates.split(",") if r]
         line["served"] = serve(ctx, tables, a.serve_requests, rates[0])
         line["served_at_rates"] = [line["served"]] + [serve(ctx, tables, a.serve_requests, r)
@@ -623,6 +636,7 @@ def main(argv=None):
                            "and waits for pool memory; throughput at saturation")
         line["served_nominal_rate"] = nominal
     if not a.no_cpu:
+        progress("CPU baseline")
         cb, ovals = cpu_baseline(tables, reqs, a.cpu_seconds)
         line["cpu_baseline"] = cb
         # the same integrals from the FP64 batched step against the oracle

@@ -1,0 +1,69 @@
+"""The batched fit sweep (the HOD-sweep workload) against the oracle: the
+first 8 of its 256 prior HOD vectors (uniform prior, seed 20220406) x the 55
+triplets (a_i, a_j, a_j) x NNMap 1h/2h/3h at tol 1e-4 and maxEval 5e6 -- the
+workload's full shape per HOD set, 1320 integrals -- submitted at once to one
+server whose region pool holds only part of them, so integrals wait for pool
+segments to free up and are admitted while others refine.  Every integral is
+compared with the oracle's value (tests/golden/hod_sweep8_oracle.npz): the 1-
+and 2-halo terms to rounding with the oracle's integrand-call counts, the
+3-halo terms within 10 % of the oracle's error estimate (for prior HOD sets
+these capped 7-dimensional integrals are more sensitive to last-place
+differences than the fiducial set's: see the full-size parity test)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, T, say
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(ROOT, "tests", "golden", "hod_sweep8_oracle.npz")
+
+
+def test_hod_sweep_first_8_sets_match_oracle(tables):
+    from g3lhalo_amd import _lib
+    g = np.load(GOLD)
+    assert str(g["tables_checksum"]) == T.checksum(tables)
+    hods = g["hods"]
+    assert np.array_equal(hods, T.hod_prior_draws(256, seed=20220406)[:8])
+    trips = T.triplets_55()
+    c = _lib.Context(tables)
+    try:
+        first = c.hodsets(hods, hods)
+        # 2^22 slots: a few hundred of these integrals in flight at a time
+        srv = _lib.Server(c, pool_slots=1 << 22, max_integrals=1 << 14)
+        res = {}
+        waited = 0
+        try:
+            for h in range(len(hods)):
+                reqs = [_lib.make_request(0, term, th * T.ARCMIN_IN_RAD, hodset=first + h,
+                                          type1=1, type2=2, reltol=1e-4, maxeval=5000000)
+                        for th in trips for term in (1, 2, 3)]
+                srv.submit(reqs, tag=h)
+            say(f"hod sweep: {len(hods)} HOD sets x {len(trips)} triplets x 3 terms submitted")
+            while len(res) < len(hods):
+                st = srv.step()
+                waited = max(waited, st.pending)
+                res.update({tag: r for tag, r in srv.poll()})
+        finally:
+            srv.close()
+    finally:
+        c.close()
+    assert waited > 0   # the bounded pool made integrals wait for admission
+    out = [r for h in range(len(hods)) for r in res[h]]
+    v = np.array([r.value for r in out])
+    rounds = np.array([r.rounds for r in out])
+    ne = np.array([r.neval for r in out])
+    d = np.abs(v - g["value"]) / np.abs(g["value"])
+    err_value = np.abs(g["err"] * g["value"] / g["raw"])
+    for term in (1, 2, 3):
+        dt = d[term - 1::3]
+        print(f"term {term}: median {np.median(dt):.2e} max {dt.max():.2e} above 1e-4: "
+              f"{(dt > 1e-4).sum()} of {len(dt)}; call counts equal "
+              f"{(rounds == g['ncalls'])[term - 1::3].sum()}")
+    assert np.array_equal(ne, g["neval"])
+    for term in (1, 2):
+        assert d[term - 1::3].max() < 1e-10
+        assert np.array_equal(rounds[term - 1::3], g["ncalls"][term - 1::3])
+    assert np.all(np.abs(v - g["value"]) < 0.1 * err_value)
