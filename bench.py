@@ -374,7 +374,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="nnmap550")
-    ap.add_argument("--plain-sample", type=int, default=6)
+    ap.add_argument("--plain-sample", type=int, default=30,
+                    help="integrals of the workload timed through the plain form")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
