@@ -2,18 +2,21 @@
 the 1650-integral NNMap workload (550 aperture triplets, fiducial HOD, types
 (1,2), tol 1e-4, maxEval 5e6) recomputed with its sensitivity probe on: every
 exp, log, pow, sin, cos and erf result inside the integrand moved by at most
-one unit in the last place (seeds 1 to 4), which is what a second, equally
-accurate math library does to the same algorithm -- except the
-nearest-index chains, which any implementation that matches the reference
-has to reproduce bit for bit.  Seeds 1-4; seeds given on the command line
-are (re)computed and merged into an existing fixture.
+one unit in the last place, which is what a second, equally accurate math
+library does to the same algorithm -- except the nearest-index chains, which
+any implementation that matches the reference has to reproduce bit for bit.
+Seeds 1-4 perturb the 3-halo integrand only; seeds 5-8 perturb the n_bar(z)
+integrals of the HOD set as well (set before the HODs are set), as a second
+implementation computes n_bar with its own library too; the fixture's
+`nbar_perturbed` records which.  Seeds given on the command line are
+(re)computed and merged into an existing fixture.
 
 Every 3-halo integral of this workload stops at the evaluation cap
 unconverged, so its value depends on which near-tied regions were split.  The
 spread between these values and the unperturbed oracle's
 (tests/golden/nnmap550_oracle.npz) is the scale on which any second
 implementation of the reference's algorithm -- the reference's own build
-against another libm included -- can agree with it.  About 35 minutes on 8
+against another libm included -- can agree with it.  About 14 minutes on 7
 cores per seed.
 
 Usage: OMP_NUM_THREADS=8 python tests/golden/make_nnmap550_ulp.py [seed ...]
@@ -31,7 +34,8 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
 import oracle as O  # noqa: E402
 from g3lhalo_amd import tables as T  # noqa: E402
 
-SEEDS = (1, 2, 3, 4)
+SEEDS = (1, 2, 3, 4, 5, 6, 7, 8)
+NBAR_SEEDS = (5, 6, 7, 8)   # n_bar(z) perturbed as well
 
 
 def main():
@@ -44,12 +48,13 @@ def main():
             have[int(seed)] = np.stack([old["value"][s], old["err"][s], old["neval"][s]], axis=1)
     O.build()
     t = T.make_tables()
-    m = O.Model(t)
-    m.set_hods(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
     trips = T.triplets_550()
     t0 = time.time()
     try:
         for seed in seeds:
+            O.set_ulp_seed(seed if seed in NBAR_SEEDS else 0)
+            m = O.Model(t)
+            m.set_hods(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
             O.set_ulp_seed(seed)
             out = np.zeros((len(trips), 3))
             for i, th in enumerate(trips):
@@ -63,7 +68,9 @@ def main():
     keys = sorted(have)
     out = np.stack([have[k] for k in keys])
     np.savez_compressed(path, seeds=np.array(keys), value=out[:, :, 0], err=out[:, :, 1],
-                        neval=out[:, :, 2].astype(np.int64), tables_checksum=T.checksum(t))
+                        neval=out[:, :, 2].astype(np.int64),
+                        nbar_perturbed=np.array([k in NBAR_SEEDS for k in keys]),
+                        tables_checksum=T.checksum(t))
 
 
 if __name__ == "__main__":

@@ -5,20 +5,29 @@ workload's full shape per HOD set, 1320 integrals -- submitted at once to one
 server whose region pool holds only part of them, so integrals wait for pool
 segments to free up and are admitted while others refine.  Every integral is
 compared with the oracle's value (tests/golden/hod_sweep8_oracle.npz): the 1-
-and 2-halo terms to rounding with the oracle's integrand-call counts, the
-3-halo terms within 10 % of the oracle's error estimate (for prior HOD sets
-these capped 7-dimensional integrals are more sensitive to last-place
-differences than the fiducial set's: see the full-size parity test)."""
+and 2-halo terms to rounding with the oracle's integrand-call counts.  The
+capped 7-dimensional 3-halo integrals refine chaotically at the last place
+(see the full-size parity test), more so for prior HOD sets than for the
+fiducial one: the oracle itself, with every exp, log, pow, sin, cos and erf
+moved by at most one unit in the last place (four seeds,
+tests/golden/hod_sweep8_oracle_ulp.npz), moves 23 to 31 of these 440
+integrals by more than 1e-4 per seed (42 distinct, up to 5.9e-3).  Their bar
+is the full-size test's (tests/ulp_bar.py): within 1e-4 of the oracle or
+within twice (to 1 %) the oracle's own largest one-ulp move on that integral, within
+10 % of the oracle's error estimate, and in aggregate no more, and no larger,
+moves beyond 1e-4 than one of the oracle's own one-ulp runs."""
 import os
 
 import numpy as np
 import pytest
 
 from conftest import ROOT, T, say
+from ulp_bar import check_three_halo
 
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(ROOT, "tests", "golden", "hod_sweep8_oracle.npz")
+ULP = os.path.join(ROOT, "tests", "golden", "hod_sweep8_oracle_ulp.npz")
 
 
 def test_hod_sweep_first_8_sets_match_oracle(tables):
@@ -67,3 +76,6 @@ def test_hod_sweep_first_8_sets_match_oracle(tables):
         assert d[term - 1::3].max() < 1e-10
         assert np.array_equal(rounds[term - 1::3], g["ncalls"][term - 1::3])
     assert np.all(np.abs(v - g["value"]) < 0.1 * err_value)
+    u = np.load(ULP)
+    assert str(u["tables_checksum"]) == str(g["tables_checksum"])
+    check_three_halo(v[2::3], g["value"][2::3], u["value"], "sweep 3-halo")

@@ -49,3 +49,40 @@ def test_ulp_fixture_entries_recompute(tables):
     finally:
         O.set_ulp_seed(0)
     assert r1["value"] == u["value"][0][i] and r1["neval"] == u["neval"][0][i]
+
+
+SWEEP = os.path.join(ROOT, "tests", "golden", "hod_sweep8_oracle.npz")
+SWEEP_ULP = os.path.join(ROOT, "tests", "golden", "hod_sweep8_oracle_ulp.npz")
+
+
+def test_ulp_spread_of_the_sweep_3h(tables):
+    """The same probe over the 8-HOD sweep fixture's 440 3-halo integrals
+    (tests/golden/hod_sweep8_oracle_ulp.npz): prior HOD sets move more of
+    them than the fiducial set does."""
+    g, u = np.load(SWEEP), np.load(SWEEP_ULP)
+    assert str(g["tables_checksum"]) == T.checksum(tables) == str(u["tables_checksum"])
+    v0 = g["value"][2::3]
+    assert u["value"].shape == (len(u["seeds"]), len(v0)) and len(u["seeds"]) >= 4
+    du = np.abs(u["value"] - v0) / np.abs(v0)
+    moved = (du > 1e-4).sum(axis=1)
+    assert 1 <= moved.min() and moved.max() <= 60
+    assert du.max() < 1e-2
+    assert np.median(du) < 1e-5
+
+
+def test_sweep_ulp_fixture_entry_recomputes(tables):
+    g, u = np.load(SWEEP), np.load(SWEEP_ULP)
+    trips = T.triplets_55()
+    du = np.abs(u["value"][0] - g["value"][2::3]) / np.abs(g["value"][2::3])
+    i = int(np.argmax(du))
+    h, t = divmod(i, len(trips))
+    th = trips[t] * T.ARCMIN_IN_RAD
+    try:
+        # the fixture's seeds perturb each HOD set's n_bar(z) integrals too
+        O.set_ulp_seed(int(u["seeds"][0]))
+        m = O.Model(tables)
+        m.set_hods(g["hods"][h], g["hods"][h])
+        r1 = m.eval_term(0, 3, th, 1, 2, 1e-4, 5000000)
+    finally:
+        O.set_ulp_seed(0)
+    assert r1["value"] == u["value"][0][i]
