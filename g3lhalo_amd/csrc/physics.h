@@ -14,13 +14,14 @@
 //          values, products, the NFW normalisation).
 //
 // Deviations of P64 from the reference that change only the last bits:
-// u_NFW by the cancellation-free branch identities noted at P64::u, with
-// Si/Ci's polynomials summed over precomputed powers and sin/cos by our own
-// Cody-Waite/Taylor routine; the HOD logarithms and powers from
-// log(m/mmin); cbrt for pow(., 1/3); exp10 for the 10^x of l1, l2; the
-// redshift-only projection factor multiplied into B in one step; shared
-// sin/cos of one argument; sub-expressions the reference evaluates
-// repeatedly with equal arguments evaluated once.  The NNMap 3-halo term,
+// u_NFW by the cancellation-free branch identities noted at u_nfw, with
+// Si/Ci's polynomials summed over precomputed powers, sin/cos by our own
+// Cody-Waite/Taylor routine and the ln x of Ci(x) by fdlibm's method
+// (log_pos); the HOD logarithms and powers from log(m/mmin); cbrt for
+// pow(., 1/3); exp10 for the 10^x of l1, l2; the redshift-only projection
+// factor multiplied into B in one step; shared sin/cos of one argument;
+// sub-expressions the reference evaluates repeatedly with equal arguments
+// evaluated once.  The NNMap 3-halo term,
 // whose integrals are sensitive to last-bit differences, keeps the
 // reference's HOD forms (P64R) and its order for the projection factor.
 // Masses keep pow(10, x) and the index chains keep the reference's exact
@@ -56,6 +57,38 @@ G3L_HD double fdiv(double a, double b) {
 #else
   return a / b;
 #endif
+}
+
+// Natural logarithm of a positive, finite, normal x, by the method of fdlibm's
+// e_log.c (Sun Microsystems, 1993): x = 2^k (1 + f) with 1 + f in
+// [sqrt(2)/2, sqrt(2)), s = f / (2 + f), log(1 + f) = 2 atanh(s)
+// = f - (f^2/2 - s (f^2/2 + R)) with R(s^2) the degree-14 minimax polynomial
+// of that method, and k ln 2 added in two parts; error below one unit in the
+// last place.  About 40 % of the library's instructions, which also handle
+// zero, negative, infinite and denormal arguments.  Used for the Ci(x) of
+// u_NFW, whose arguments are in (0, 4); the look-up indices keep the
+// library's log.
+G3L_HD double log_pos(double x) {
+  int k;
+  double m = frexp(x, &k);   // x = m 2^k, m in [1/2, 1)
+  if (m < 0.70710678118654752440) {
+    m += m;
+    k -= 1;
+  }
+  const double f = m - 1.0;
+  const double s = fdiv(f, 2.0 + f);
+  const double z = s * s, w = z * z;
+  const double t1 = w * fma(w, fma(w, 1.531383769920937332e-01, 2.222219843214978396e-01),
+                            3.999999999940941908e-01);
+  const double t2 =
+      z * fma(w, fma(w, fma(w, 1.479819860511658591e-01, 1.818357216161805012e-01),
+                     2.857142874366239149e-01),
+              6.666666666666735130e-01);
+  const double R = t2 + t1;
+  const double hfsq = 0.5 * f * f;
+  const double dk = (double)k;
+  return dk * 6.93147180369123816490e-01 -
+         ((hfsq - (s * (hfsq + R) + dk * 1.90821492927058770002e-10)) - f);
 }
 
 // sin and cos together for |x| < 2^20; larger arguments take the library's
@@ -389,6 +422,49 @@ G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
 }
 
 // ======================================================================
+// Truncated-NFW transform (src/generalFunctions.cu:6-30),
+//   u F = sin a1 (Si a2 - Si a1) + cos a1 (Ci a2 - Ci a1) - sin(a1 c) / a2,
+// a1 = k r200 / c, a2 = a1 (1 + c), F = ln(1+c) - c/(1+c), rearranged
+// (equal in exact arithmetic) by the branch of the Si/Ci approximation
+// each argument falls in:
+//   a1 >= 4:       u F = g(a1) + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c);
+//   a1 < 4 <= a2:  u F = pi/2 sin a1 - sin a1 Si(a1) - cos a1 Ci(a1)
+//                        + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c);
+//   a2 < 4:        u = 1 + [(cos a1 - 1) ln(1+c) + (a1 c - sin a1 c) / a2
+//                        + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si a2 - Si a1)] / F,
+// with Ci = gamma + ln x + Cr below 4.  Every branch uses sin/cos of a1 c
+// (a2 - a1) instead of a2, none has the reference's O(1) cancellations,
+// and the pieces two branches share are computed once, so a wave whose
+// lanes fall in different branches runs each piece at most once.
+template <class P>
+G3L_HD double u_nfw(const Nfw &n, double k) {
+  const double a1 = fdiv(k * n.r200, n.c);
+  const double a2 = a1 * (1 + n.c);
+  double sx, cx;
+  sincos_cw(a1 * n.c, sx, cx);
+  double s1 = 0, c1 = 0, si1 = 0, cr1 = 0, g2 = 0, fm2 = 0;
+  if (a1 < 4) {
+    sincos_cw(a1, s1, c1);
+    sici_small_parts(a1, si1, cr1);
+  }
+  if (a2 >= 4) sici_aux_gfm(a2, g2, fm2);
+  if (a1 >= 4) {
+    double g1, fm1;
+    sici_aux_gfm(a1, g1, fm1);
+    return fdiv(g1 + sx * fm2 - g2 * cx, n.F);
+  }
+  if (a2 >= 4) {
+    const double ci1 = 0.5772156649 + P::ln_ci(a1) + cr1;
+    return fdiv(0.5 * 3.14159265358979323846 * s1 - s1 * si1 - c1 * ci1 + sx * fm2 - g2 * cx,
+                n.F);
+  }
+  double si2, cr2;
+  sici_small_parts(a2, si2, cr2);
+  const double rest =
+      (c1 - 1) * n.lc + fdiv(a1 * n.c - sx, a2) + c1 * (cr2 - cr1) + s1 * (si2 - si1);
+  return 1 + fdiv(rest, n.F);
+}
+
 // FP64 policy
 struct P64 {
   static constexpr bool kMixed = false;
@@ -412,53 +488,17 @@ struct P64 {
     return exp(eps * (M.L + lmmin));
   }
 
-  // Truncated-NFW transform (src/generalFunctions.cu:6-30),
-  //   u F = sin a1 (Si a2 - Si a1) + cos a1 (Ci a2 - Ci a1) - sin(a1 c) / a2,
-  // a1 = k r200 / c, a2 = a1 (1 + c), F = ln(1+c) - c/(1+c), rearranged
-  // (equal in exact arithmetic) by the branch of the Si/Ci approximation
-  // each argument falls in:
-  //   a1 >= 4:       u F = g(a1) + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c);
-  //   a1 < 4 <= a2:  u F = pi/2 sin a1 - sin a1 Si(a1) - cos a1 Ci(a1)
-  //                        + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c);
-  //   a2 < 4:        u = 1 + [(cos a1 - 1) ln(1+c) + (a1 c - sin a1 c) / a2
-  //                        + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si a2 - Si a1)] / F,
-  // with Ci = gamma + ln x + Cr below 4.  Every branch uses sin/cos of a1 c
-  // (a2 - a1) instead of a2, none has the reference's O(1) cancellations,
-  // and the pieces two branches share are computed once, so a wave whose
-  // lanes fall in different branches runs each piece at most once.
-  G3L_HD static double u(const Nfw &n, double k) {
-    const double a1 = fdiv(k * n.r200, n.c);
-    const double a2 = a1 * (1 + n.c);
-    double sx, cx;
-    sincos_cw(a1 * n.c, sx, cx);
-    double s1 = 0, c1 = 0, si1 = 0, cr1 = 0, g2 = 0, fm2 = 0;
-    if (a1 < 4) {
-      sincos_cw(a1, s1, c1);
-      sici_small_parts(a1, si1, cr1);
-    }
-    if (a2 >= 4) sici_aux_gfm(a2, g2, fm2);
-    if (a1 >= 4) {
-      double g1, fm1;
-      sici_aux_gfm(a1, g1, fm1);
-      return fdiv(g1 + sx * fm2 - g2 * cx, n.F);
-    }
-    if (a2 >= 4) {
-      const double ci1 = 0.5772156649 + log(a1) + cr1;
-      return fdiv(0.5 * 3.14159265358979323846 * s1 - s1 * si1 - c1 * ci1 + sx * fm2 - g2 * cx,
-                  n.F);
-    }
-    double si2, cr2;
-    sici_small_parts(a2, si2, cr2);
-    const double rest =
-        (c1 - 1) * n.lc + fdiv(a1 * n.c - sx, a2) + c1 * (cr2 - cr1) + s1 * (si2 - si1);
-    return 1 + fdiv(rest, n.F);
-  }
+  // Truncated-NFW transform (u_nfw above); ln x of Ci(x) by log_pos.
+  G3L_HD static double ln_ci(double x) { return log_pos(x); }
+  G3L_HD static double u(const Nfw &n, double k);
 
   // U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42).
   G3L_HD static double ap(double eta) {
     return 0.5 * eta * eta * exp(-0.5 * eta * eta);
   }
 };
+
+G3L_HD double P64::u(const Nfw &n, double k) { return u_nfw<P64>(n, k); }
 
 // FP64 with the reference's own expressions for 10^x of l1, l2 and the r_200
 // cube root (pow), the HOD occupations and NsatNsat's power
@@ -489,6 +529,9 @@ struct P64R : P64 {
     (void)lmmin;
     return pow(M.m, eps);
   }
+  // the library's log in Ci(x)
+  G3L_HD static double ln_ci(double x) { return log(x); }
+  G3L_HD static double u(const Nfw &n, double k) { return u_nfw<P64R>(n, k); }
 };
 
 // u_NFW for a2 < 4 (small k r_s), the FP64 branch of P64::u on its own:
