@@ -476,7 +476,7 @@ template <int KIND, class P, int I>
 __device__ __forceinline__ bool u_inputs(const DevTables &T,
                                          const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
                                          const Cache<KIND> &C, int i, int ng, int &dst, Nfw &n,
-                                         double &k) {
+                                         double &k, uint32_t &nfe) {
   using S = Spec<kind_of_kv(KIND)>;
   if constexpr (I >= S::NU) {
     return false;
@@ -484,7 +484,7 @@ __device__ __forceinline__ bool u_inputs(const DevTables &T,
     constexpr int D = S::D;
     using F = UF<KIND, I>;
     constexpr int lo = UOff<KIND, I>::v, hi = UOff<KIND, I + 1>::v;
-    if (i >= hi) return u_inputs<KIND, P, I + 1>(T, St, C, i, ng, dst, n, k);
+    if (i >= hi) return u_inputs<KIND, P, I + 1>(T, St, C, i, ng, dst, n, k, nfe);
     const int r = i - lo, g = r / F::n, c = r - g * F::n;
     if (g >= ng || St.slot[g] == 0xffffffffu) return false;
     const int q = kProj<D, F::s.mask>.rep[c];
@@ -506,6 +506,8 @@ __device__ __forceinline__ bool u_inputs(const DevTables &T,
     if constexpr (cache_nfw<KIND>()) {
       const NfwC &en = C.nf[F::s.slot][g][cm];
       n = Nfw{e.r200, en.c[F::s.fsel], en.lc[F::s.fsel], en.F[F::s.fsel]};
+      // the normalisations' entry in C.nf (flat), times 4, plus the f selector
+      nfe = (uint32_t)(((F::s.slot * KGroup<KIND>::G + g) * 17 + cm) * 4 + F::s.fsel);
     } else {
       Mass M;
       M.m = e.m;
@@ -534,21 +536,14 @@ struct P2Layout {
   static constexpr int END = PK3 + (S::has_pk ? pad(G * Cache<KIND>::N3Z) : 0);
 };
 
+// Pass 2 items after the u_NFW ones (i >= the u items' padded count).
 template <int KIND, class P>
-__device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
-                                      Cache<KIND> &C, int i, int ng) {
+__device__ __forceinline__ void pass2_rest(const DevTables &T, const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
+                                           Cache<KIND> &C, int i, int ng) {
   using S = Spec<kind_of_kv(KIND)>;
   constexpr int D = S::D, G = KGroup<KIND>::G;
   constexpr int NU_TOTAL = Cache<KIND>::NU_TOTAL;
   constexpr int UPAD = (NU_TOTAL + 63) / 64 * 64;
-  if (i < NU_TOTAL) {
-    int dst;
-    Nfw n;
-    double k;
-    if (!u_inputs<KIND, P, 0>(T, St, C, i, ng, dst, n, k)) return;
-    C.u[dst] = P::u(n, k);
-    return;
-  }
   using Lay = P2Layout<KIND>;
   if (i < Lay::L3) {
     if constexpr (S::has_bilin) {
@@ -594,6 +589,108 @@ __device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<kind_
   }
 }
 
+template <int KIND, class P>
+__device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
+                                      Cache<KIND> &C, int i, int ng) {
+  if (i < Cache<KIND>::NU_TOTAL) {
+    int dst;
+    Nfw n;
+    double k;
+    uint32_t nfe;
+    if (!u_inputs<KIND, P, 0>(T, St, C, i, ng, dst, n, k, nfe)) return;
+    C.u[dst] = P::u(n, k);
+    return;
+  }
+  pass2_rest<KIND, P>(T, St, C, i, ng);
+}
+
+// u_NFW items in branch order.  u_NFW has three branches (a1 >= 4,
+// a1 < 4 <= a2, a2 < 4) and a wave runs every branch one of its lanes takes:
+// on the 550-triplet workload about half of the waves of u items mix two or
+// three branches and run about a third more instructions than their lanes
+// need.  For the FP64 policies of the kinds that cache the NFW
+// normalisations, pass 2 therefore (1) computes each u item's a1, keeps it
+// in the item's own output slot and appends the item to its branch's list
+// (one LDS atomic per wave), and (2) evaluates the items from the lists in
+// branch order, so that a wave's lanes mostly share one branch.  Every item runs the same expressions on
+// the same inputs either way: the values are bit-identical.
+template <int KIND>
+constexpr bool sort_u(bool mixed) { return cache_nfw<KIND>() && !mixed; }
+
+// Step 1 for u item i: a1 into C.u[i], its normalisations' entry into
+// unf[i], and i into its branch's list: a1 >= 4 from the front of `ends`,
+// a2 < 4 from the back of `ends`, a1 < 4 <= a2 from the front of `mid`.  The
+// three branch counts of a wave are added to the block's counters (21-bit
+// fields of one 64-bit word) by a single LDS atomic, whose old value gives
+// each branch's base.
+template <int KIND, class P>
+__device__ __forceinline__ void u_classify(const DevTables &T,
+                                           const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
+                                           Cache<KIND> &C, int i, int ng, unsigned long long *cnt,
+                                           uint16_t *unf, uint16_t *ends, uint16_t *mid) {
+  constexpr int NU = Cache<KIND>::NU_TOTAL;
+  int dst = 0;
+  Nfw n;
+  double k = 0;
+  uint32_t nfe = 0;
+  const bool ok = u_inputs<KIND, P, 0>(T, St, C, i, ng, dst, n, k, nfe);
+  int b = 3;
+  if (ok) {
+    const double a1 = nfw_a1(n, k);
+    C.u[dst] = a1;
+    unf[dst] = (uint16_t)nfe;
+    b = a1 >= 4 ? 0 : (a1 * (1 + n.c) >= 4 ? 1 : 2);
+  }
+  const uint64_t m0 = __ballot(b == 0), m1 = __ballot(b == 1), m2 = __ballot(b == 2);
+  const uint64_t add = (uint64_t)__popcll(m0) | ((uint64_t)__popcll(m1) << 21) |
+                       ((uint64_t)__popcll(m2) << 42);
+  const int first = __ffsll((unsigned long long)__ballot(1)) - 1;
+  uint64_t old = 0;
+  if ((int)__lane_id() == first) old = atomicAdd(cnt, (unsigned long long)add);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)old, first);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(old >> 32), first);
+  if (b == 3) return;
+  const uint64_t o = ((uint64_t)hi << 32) | lo;
+  const uint32_t base = (uint32_t)(o >> (21 * b)) & 0x1fffffu;
+  const uint64_t m = b == 0 ? m0 : b == 1 ? m1 : m2;
+  const uint32_t pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  if (b == 1) mid[pos] = (uint16_t)dst;
+  else ends[b == 0 ? pos : NU - 1 - pos] = (uint16_t)dst;
+}
+
+// Step 1, round r of R over the pass-2 items tid + r NT: each round holds
+// only the code its items can reach (u items, the rest, or both).
+template <int KIND, class P, int r, int R>
+__device__ __forceinline__ void sort_round(const DevTables &T,
+                                           const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
+                                           Cache<KIND> &C, int tid, int ng, unsigned long long *cnt,
+                                           uint16_t *unf, uint16_t *ends, uint16_t *mid) {
+  if constexpr (r < R) {
+    constexpr int NT = KGroup<KIND>::NT, NU = Cache<KIND>::NU_TOTAL, P2 = P2Layout<KIND>::END;
+    const int i = tid + r * NT;
+    if constexpr ((r + 1) * NT <= NU) {
+      u_classify<KIND, P>(T, St, C, i, ng, cnt, unf, ends, mid);
+    } else if constexpr (r * NT >= NU) {
+      if (i < P2) pass2_rest<KIND, P>(T, St, C, i, ng);
+    } else {
+      if (i < NU) u_classify<KIND, P>(T, St, C, i, ng, cnt, unf, ends, mid);
+      else if (i < P2) pass2_rest<KIND, P>(T, St, C, i, ng);
+    }
+    sort_round<KIND, P, r + 1, R>(T, St, C, tid, ng, cnt, unf, ends, mid);
+  }
+}
+
+// Step 3 for u item i, whose a1 is in C.u[i] and normalisations' entry in
+// unf[i].
+template <int KIND, class P>
+__device__ __forceinline__ void u_eval_sorted(Cache<KIND> &C, int i, const uint16_t *unf) {
+  const uint32_t e = unf[i];
+  const NfwC &en = (&C.nf[0][0][0])[e >> 2];
+  const int fs = e & 3;
+  C.u[i] = P::u_a1(C.u[i], Nfw{0.0, en.c[fs], en.lc[fs], en.F[fs]});
+}
+
 // At most 64 VGPRs (8 waves per SIMD): the FP64 5-D and 7-D kinds otherwise
 // compile to 70-80 and run 6-7 waves per SIMD.  The few registers this
 // spills cost less than the lost occupancy: NNMap 1h 4.12 -> 3.69 ms and
@@ -609,20 +706,49 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   constexpr int P2 = P2Layout<KIND>::END;
   static_assert(P2 <= 3 * NT, "pass 2 fits three items per thread");
   static_assert(G * NP <= NT, "pass 3 fits one point per thread");
+  using Q = typename EvalPolicy<kind_of_kv(KIND), P>::type;
+  constexpr bool kSort = sort_u<KIND>(Q::kMixed);
+  constexpr int NU = Cache<KIND>::NU_TOTAL;
+  // s_f holds the point values of pass 3, and before that the branch lists
+  // of the u items (2 NU entries of 16 bits)
+  constexpr int NF = kSort && (4 * NU + 7) / 8 > G * NP ? (4 * NU + 7) / 8 : G * NP;
   __shared__ Stage<D, G> St;
   __shared__ Cache<KIND> C;
-  __shared__ double s_f[G * NP];
-  using Q = typename EvalPolicy<kind_of_kv(KIND), P>::type;
+  __shared__ double s_f[NF];
+  __shared__ unsigned long long s_ucnt;   // u items per branch, 21-bit fields
+  __shared__ uint16_t s_unf[kSort ? NU : 1];
   if (blockIdx.x * G >= L.count) return;
   const int ng = stage_regions<D, G, NT>(L, St);
   const int tid = threadIdx.x;
   const DevTables T = L.T;
+  if (kSort && tid == 0) s_ucnt = 0;
   pass1<KIND, Q>(T, St, C, tid, ng);
   if constexpr (P1Layout<KIND>::END > NT) pass1<KIND, Q>(T, St, C, tid + NT, ng);
   __syncthreads();
-  pass2<KIND, Q>(T, St, C, tid, ng);
-  if constexpr (P2 > NT) pass2<KIND, Q>(T, St, C, tid + NT, ng);
-  if constexpr (P2 > 2 * NT) pass2<KIND, Q>(T, St, C, tid + 2 * NT, ng);
+  if constexpr (kSort) {
+    constexpr int R = (P2 + NT - 1) / NT;
+    uint16_t *ends = reinterpret_cast<uint16_t *>(s_f), *mid = ends + NU;
+    sort_round<KIND, Q, 0, R>(T, St, C, tid, ng, &s_ucnt, s_unf, ends, mid);
+    __syncthreads();
+    const unsigned long long cw = s_ucnt;
+    const uint32_t nb = (uint32_t)cw & 0x1fffffu, nbm = nb + ((uint32_t)(cw >> 21) & 0x1fffffu);
+    const uint32_t nu = nbm + (uint32_t)(cw >> 42);
+    // fixed rounds rather than a loop: in a loop the compiler keeps the
+    // polynomial coefficients in registers across iterations and spills them
+    constexpr int RU = (NU + NT - 1) / NT;
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      const uint32_t j = tid + r * NT;
+      if (j < nu) {
+        const uint16_t item = j < nb ? ends[j] : j < nbm ? mid[j - nb] : ends[NU - 1 - (j - nbm)];
+        u_eval_sorted<KIND, Q>(C, item, s_unf);
+      }
+    }
+  } else {
+    pass2<KIND, Q>(T, St, C, tid, ng);
+    if constexpr (P2 > NT) pass2<KIND, Q>(T, St, C, tid + NT, ng);
+    if constexpr (P2 > 2 * NT) pass2<KIND, Q>(T, St, C, tid + 2 * NT, ng);
+  }
   __syncthreads();
   if (tid < ng * NP) {
     const int g = tid / NP, q = tid - g * NP;

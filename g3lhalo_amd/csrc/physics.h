@@ -436,9 +436,11 @@ G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
 // (a2 - a1) instead of a2, none has the reference's O(1) cancellations,
 // and the pieces two branches share are computed once, so a wave whose
 // lanes fall in different branches runs each piece at most once.
+//
+// u_nfw_a1 takes a1 itself (for callers that computed it to sort items by
+// branch); u_nfw computes it first, by the same expression.
 template <class P>
-G3L_HD double u_nfw(const Nfw &n, double k) {
-  const double a1 = fdiv(k * n.r200, n.c);
+G3L_HD double u_nfw_a1(double a1, const Nfw &n) {
   const double a2 = a1 * (1 + n.c);
   double sx, cx;
   sincos_cw(a1 * n.c, sx, cx);
@@ -463,6 +465,12 @@ G3L_HD double u_nfw(const Nfw &n, double k) {
   const double rest =
       (c1 - 1) * n.lc + fdiv(a1 * n.c - sx, a2) + c1 * (cr2 - cr1) + s1 * (si2 - si1);
   return 1 + fdiv(rest, n.F);
+}
+// a1 = k r200 / c of u_NFW.
+G3L_HD double nfw_a1(const Nfw &n, double k) { return fdiv(k * n.r200, n.c); }
+template <class P>
+G3L_HD double u_nfw(const Nfw &n, double k) {
+  return u_nfw_a1<P>(nfw_a1(n, k), n);
 }
 
 // FP64 policy
@@ -491,6 +499,7 @@ struct P64 {
   // Truncated-NFW transform (u_nfw above); ln x of Ci(x) by log_pos.
   G3L_HD static double ln_ci(double x) { return log_pos(x); }
   G3L_HD static double u(const Nfw &n, double k);
+  G3L_HD static double u_a1(double a1, const Nfw &n);
 
   // U(eta) = eta^2/2 exp(-eta^2/2) (src/generalFunctions.cu:39-42).
   G3L_HD static double ap(double eta) {
@@ -499,6 +508,7 @@ struct P64 {
 };
 
 G3L_HD double P64::u(const Nfw &n, double k) { return u_nfw<P64>(n, k); }
+G3L_HD double P64::u_a1(double a1, const Nfw &n) { return u_nfw_a1<P64>(a1, n); }
 
 // FP64 with the reference's own expressions for 10^x of l1, l2 and the r_200
 // cube root (pow), the HOD occupations and NsatNsat's power
@@ -532,6 +542,7 @@ struct P64R : P64 {
   // the library's log in Ci(x)
   G3L_HD static double ln_ci(double x) { return log(x); }
   G3L_HD static double u(const Nfw &n, double k) { return u_nfw<P64R>(n, k); }
+  G3L_HD static double u_a1(double a1, const Nfw &n) { return u_nfw_a1<P64R>(a1, n); }
 };
 
 // u_NFW for a2 < 4 (small k r_s), the FP64 branch of P64::u on its own:
