@@ -16,8 +16,9 @@ last place: their error estimates are 3 % to 70x their values, their
 squeezed bispectrum configurations (k3 -> 0) amplify rounding, and a
 last-place change in any integrand factor can change which of two near-tied
 regions is split.  The oracle itself, with its math library moved by one
-unit in the last place (tests/golden/nnmap550_oracle_ulp.npz), moves 3 to 8
-of the 550 by more than 1e-4 per seed.  Their bar (tests/ulp_bar.py), per
+unit in the last place (tests/golden/nnmap550_oracle_ulp.npz, eight seeds,
+four of them perturbing the n_bar(z) integrals too), moves 3 to 15 of the
+550 by more than 1e-4 per seed (26 distinct, up to 5.3e-3).  Their bar (tests/ulp_bar.py), per
 integral: within 1e-4 of the oracle or within twice (to 1 %) the oracle's own largest
 one-ulp move on that integral, and within 10 % of the oracle's error
 estimate; in aggregate no more, and no larger, moves beyond 1e-4 than one of

@@ -33,6 +33,24 @@ def test_ulp_spread_of_the_3h_workload(tables):
     assert np.median(du) < 1e-7
 
 
+def test_ulp_fixture_nbar_seed_recomputes(tables):
+    """A seed that perturbs the n_bar(z) integrals as well: the seed is set
+    before the HOD set is."""
+    g, u = np.load(GOLD), np.load(ULP)
+    s = int(np.flatnonzero(u["nbar_perturbed"])[0])
+    du = np.abs(u["value"][s] - g["value"][2::3]) / np.abs(g["value"][2::3])
+    i = int(np.argmax(du))
+    th = T.triplets_550()[i] * T.ARCMIN_IN_RAD
+    try:
+        O.set_ulp_seed(int(u["seeds"][s]))
+        m = O.Model(tables)
+        m.set_hods(T.FIDUCIAL_HOD, T.FIDUCIAL_HOD)
+        r = m.eval_term(0, 3, th, 1, 2, 1e-4, 5000000)
+    finally:
+        O.set_ulp_seed(0)
+    assert r["value"] == u["value"][s][i] and r["neval"] == u["neval"][s][i]
+
+
 def test_ulp_fixture_entries_recompute(tables):
     g, u = np.load(GOLD), np.load(ULP)
     trips = T.triplets_550()
