@@ -608,14 +608,13 @@ __device__ __forceinline__ void pass2(const DevTables &T, const Stage<Spec<kind_
 // a1 < 4 <= a2, a2 < 4) and a wave runs every branch one of its lanes takes:
 // on the 550-triplet workload about half of the waves of u items mix two or
 // three branches and run about a third more instructions than their lanes
-// need.  For the FP64 policies of the kinds that cache the NFW
-// normalisations, pass 2 therefore (1) computes each u item's a1, keeps it
+// need.  For the kinds that cache the NFW normalisations, pass 2 therefore (1) computes each u item's a1, keeps it
 // in the item's own output slot and appends the item to its branch's list
 // (one LDS atomic per wave), and (2) evaluates the items from the lists in
 // branch order, so that a wave's lanes mostly share one branch.  Every item runs the same expressions on
 // the same inputs either way: the values are bit-identical.
 template <int KIND>
-constexpr bool sort_u(bool mixed) { return cache_nfw<KIND>() && !mixed; }
+constexpr bool sort_u() { return cache_nfw<KIND>(); }
 
 // Step 1 for u item i: a1 into C.u[i], its normalisations' entry into
 // unf[i], and i into its branch's list: a1 >= 4 from the front of `ends`,
@@ -707,7 +706,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   static_assert(P2 <= 3 * NT, "pass 2 fits three items per thread");
   static_assert(G * NP <= NT, "pass 3 fits one point per thread");
   using Q = typename EvalPolicy<kind_of_kv(KIND), P>::type;
-  constexpr bool kSort = sort_u<KIND>(Q::kMixed);
+  constexpr bool kSort = sort_u<KIND>();
   constexpr int NU = Cache<KIND>::NU_TOTAL;
   // s_f holds the point values of pass 3, and before that the branch lists
   // of the u items (2 NU entries of 16 bits)

@@ -677,8 +677,8 @@ struct PMix {
   //                        + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c),
   //                  with Ci = gamma + ln a1 + Cr; its O(1) terms sum to
   //                  u F ~ 0.2-2 there, so FP32 keeps ~6 digits.
-  G3L_HD static double u(const Nfw &n, double k) {
-    double a1d = fdiv(k * n.r200, n.c);
+  G3L_HD static double u(const Nfw &n, double k) { return u_a1(nfw_a1(n, k), n); }
+  G3L_HD static double u_a1(double a1d, const Nfw &n) {
     double a2d = a1d * (1 + n.c);
     // small k r_s: u = 1 - O(a1^2) from differences of nearby Si/Ci values,
     // kept in FP64
