@@ -9,13 +9,14 @@ and 2-halo terms to rounding with the oracle's integrand-call counts.  The
 capped 7-dimensional 3-halo integrals refine chaotically at the last place
 (see the full-size parity test), more so for prior HOD sets than for the
 fiducial one: the oracle itself, with every exp, log, pow, sin, cos and erf
-moved by at most one unit in the last place (four seeds,
-tests/golden/hod_sweep8_oracle_ulp.npz), moves 23 to 31 of these 440
-integrals by more than 1e-4 per seed (42 distinct, up to 5.9e-3).  Their bar
-is the full-size test's (tests/ulp_bar.py): within 1e-4 of the oracle or
-within twice (to 1 %) the oracle's own largest one-ulp move on that integral, within
-10 % of the oracle's error estimate, and in aggregate no more, and no larger,
-moves beyond 1e-4 than one of the oracle's own one-ulp runs."""
+moved by at most one unit in the last place
+(tests/golden/hod_sweep8_oracle_ulp.npz, eight seeds), moves 18 to 31 of
+these 440 integrals by more than 1e-4 per seed (45 distinct, up to 9.5e-3).
+Their bar is the full-size test's (tests/ulp_bar.py): within 1e-4 of the
+oracle or within twice (to 1 %) the oracle's own largest one-ulp move on
+that integral, within 10 % of the oracle's error estimate, and in aggregate
+no more, and no larger, moves beyond 1e-4 than one of the oracle's own
+one-ulp runs."""
 import os
 
 import numpy as np
