@@ -179,7 +179,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 
 struct MassC {
   double m, L, r200;
-  int ir, it;
+  int ir, it, zt;
   double nc1, ns1, nc2, ns2, nss;
 };
 
@@ -194,6 +194,16 @@ struct NfwC {
 // the logarithms it saves.
 template <int KIND>
 constexpr bool cache_nfw() { return Spec<kind_of_kv(KIND)>::D <= 6; }
+
+// Whether pass 1 runs in two stages (the kinds that cache the NFW
+// normalisations): the masses first, then the occupations and the
+// normalisations as items of their own.  A (mass, z) item that does all of it
+// is the pass's longest by far (pow, log, cbrt, two erf/exp occupations,
+// three normalisations), and the block's other waves wait for it at the
+// barrier; staged, the longest item is about a third as long, with one more
+// barrier and no work repeated (NNMap 1h launches 3 %, 2h 1 % faster).
+template <int KIND>
+constexpr bool staged_p1() { return Spec<kind_of_kv(KIND)>::D <= 6; }
 
 template <int KIND, int I>
 struct UF {
@@ -424,6 +434,8 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     e.r200 = M.r200;
     e.ir = M.ir;
     e.it = M.it;
+    e.zt = zt;
+    if constexpr (staged_p1<KIND>()) return;   // the rest in pass1b
     if constexpr (kind_of_kv(KIND) == kNN3) {
       // the NNMap 3-halo term needs the first HOD at m1 and the second at m2
       // only: one occupation per item, the HOD selected by the mass slot
@@ -469,6 +481,68 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     const int zr = zix_round(T, x[D - 1]);
     C.pk[which][g][c] = pk(T, zr, l / T.w[zr]);
   }
+}
+
+// Second stage of pass 1 (staged_p1 kinds): item i is a (mass, z) class's
+// HOD occupations (NM x G x 17 items), or one of its NFW normalisations (3 x
+// NM x G x 17 items, one per f selector), from the masses of the first stage.
+template <int KIND>
+struct P1bLayout {
+  using S = Spec<kind_of_kv(KIND)>;
+  static constexpr int G = KGroup<KIND>::G;
+  static constexpr int pad(int n) { return (n + 63) / 64 * 64; }
+  static constexpr int NMZ = S::NM * G * 17;
+  static constexpr int NFW = pad(NMZ), END = NFW + pad(3 * NMZ);
+};
+
+template <int KIND, class P>
+__device__ __forceinline__ void pass1b(const DevTables &T, const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
+                                       Cache<KIND> &C, int i, int ng) {
+  using S = Spec<kind_of_kv(KIND)>;
+  using Lay = P1bLayout<KIND>;
+  constexpr int G = KGroup<KIND>::G;
+  if (i < Lay::NFW) {
+    if (i >= Lay::NMZ) return;
+    const int slot = i / (G * 17), rr = i - slot * G * 17, g = rr / 17, c = rr - g * 17;
+    if (g >= ng || St.slot[g] == 0xffffffffu) return;
+    MassC &e = C.m[slot][g][c];
+    Mass M;
+    M.m = e.m;
+    M.L = e.L;
+    M.ir = e.ir;
+    M.it = e.it;
+    M.r200 = e.r200;
+    const TermArgs &a = St.args[g];
+    const Occ o1 = occ1<P>(a, M);
+    e.nc1 = o1.nc;
+    e.ns1 = o1.ns;
+    if constexpr (S::two_hods) {
+      const Occ o2 = occ2<P>(a, M);
+      e.nc2 = o2.nc;
+      e.ns2 = o2.ns;
+      if (slot == S::nss_slot) e.nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+    }
+    return;
+  }
+  const int r = i - Lay::NFW;
+  if (r >= 3 * Lay::NMZ) return;
+  const int fs = r / Lay::NMZ, rm = r - fs * Lay::NMZ;
+  const int slot = rm / (G * 17), rr = rm - slot * G * 17, g = rr / 17, c = rr - g * 17;
+  const bool need = slot == 0 ? uses_nfw<kind_of_kv(KIND)>(0, fs)
+                              : slot == 1 ? uses_nfw<kind_of_kv(KIND)>(1, fs) : uses_nfw<kind_of_kv(KIND)>(2, fs);
+  if (!need || g >= ng || St.slot[g] == 0xffffffffu) return;
+  const MassC &e = C.m[slot][g][c];
+  Mass M;
+  M.m = e.m;
+  M.L = e.L;
+  M.ir = e.ir;
+  M.it = e.it;
+  M.r200 = e.r200;
+  const Nfw nf = make_nfw(T, M, e.zt, f_of(St.args[g], fs));
+  NfwC &en = C.nf[slot][g][c];
+  en.c[fs] = nf.c;
+  en.lc[fs] = nf.lc;
+  en.F[fs] = nf.F;
 }
 
 // Pass 2: the inputs of u-factor item i (factor I or later).
@@ -724,6 +798,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   pass1<KIND, Q>(T, St, C, tid, ng);
   if constexpr (P1Layout<KIND>::END > NT) pass1<KIND, Q>(T, St, C, tid + NT, ng);
   __syncthreads();
+  if constexpr (staged_p1<KIND>()) {
+    static_assert(P1bLayout<KIND>::END <= 2 * NT, "pass 1b fits two items per thread");
+    pass1b<KIND, Q>(T, St, C, tid, ng);
+    if constexpr (P1bLayout<KIND>::END > NT) pass1b<KIND, Q>(T, St, C, tid + NT, ng);
+    __syncthreads();
+  }
   if constexpr (kSort) {
     constexpr int R = (P2 + NT - 1) / NT;
     uint16_t *ends = reinterpret_cast<uint16_t *>(s_f), *mid = ends + NU;
