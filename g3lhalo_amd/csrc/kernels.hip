@@ -483,17 +483,30 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
   }
 }
 
-// Second stage of pass 1 (staged_p1 kinds): item i is a (mass, z) class's
-// HOD occupations (NM x G x 17 items), or one of its NFW normalisations (3 x
-// NM x G x 17 items, one per f selector), from the masses of the first stage.
+// Second stage of pass 1 (staged_p1 kinds): item i is one of a (mass, z)
+// class's HOD occupations (the first HOD's, and the second's for the
+// two-HOD kinds: NM x G x 17 items each) or one of its NFW normalisations (3
+// x NM x G x 17 items, one per f selector), from the masses of the first
+// stage.  NsatNsat, which needs both occupations, is a pass-2 item.
 template <int KIND>
 struct P1bLayout {
   using S = Spec<kind_of_kv(KIND)>;
   static constexpr int G = KGroup<KIND>::G;
   static constexpr int pad(int n) { return (n + 63) / 64 * 64; }
   static constexpr int NMZ = S::NM * G * 17;
-  static constexpr int NFW = pad(NMZ), END = NFW + pad(3 * NMZ);
+  static constexpr int OCC2 = pad(NMZ), NFW = OCC2 + (S::two_hods ? pad(NMZ) : 0);
+  static constexpr int END = NFW + pad(3 * NMZ);
 };
+
+__device__ __forceinline__ Mass mass_of(const MassC &e) {
+  Mass M;
+  M.m = e.m;
+  M.L = e.L;
+  M.ir = e.ir;
+  M.it = e.it;
+  M.r200 = e.r200;
+  return M;
+}
 
 template <int KIND, class P>
 __device__ __forceinline__ void pass1b(const DevTables &T, const Stage<Spec<kind_of_kv(KIND)>::D, KGroup<KIND>::G> &St,
@@ -502,25 +515,22 @@ __device__ __forceinline__ void pass1b(const DevTables &T, const Stage<Spec<kind
   using Lay = P1bLayout<KIND>;
   constexpr int G = KGroup<KIND>::G;
   if (i < Lay::NFW) {
-    if (i >= Lay::NMZ) return;
-    const int slot = i / (G * 17), rr = i - slot * G * 17, g = rr / 17, c = rr - g * 17;
+    const bool second = i >= Lay::OCC2;
+    const int r = second ? i - Lay::OCC2 : i;
+    if (r >= Lay::NMZ) return;
+    const int slot = r / (G * 17), rr = r - slot * G * 17, g = rr / 17, c = rr - g * 17;
     if (g >= ng || St.slot[g] == 0xffffffffu) return;
     MassC &e = C.m[slot][g][c];
-    Mass M;
-    M.m = e.m;
-    M.L = e.L;
-    M.ir = e.ir;
-    M.it = e.it;
-    M.r200 = e.r200;
+    const Mass M = mass_of(e);
     const TermArgs &a = St.args[g];
-    const Occ o1 = occ1<P>(a, M);
-    e.nc1 = o1.nc;
-    e.ns1 = o1.ns;
-    if constexpr (S::two_hods) {
-      const Occ o2 = occ2<P>(a, M);
-      e.nc2 = o2.nc;
-      e.ns2 = o2.ns;
-      if (slot == S::nss_slot) e.nss = nsatnsat<P>(a, o1.ns, o2.ns, M);
+    if (!second) {
+      const Occ o = occ1<P>(a, M);
+      e.nc1 = o.nc;
+      e.ns1 = o.ns;
+    } else if constexpr (S::two_hods) {
+      const Occ o = occ2<P>(a, M);
+      e.nc2 = o.nc;
+      e.ns2 = o.ns;
     }
     return;
   }
@@ -532,13 +542,7 @@ __device__ __forceinline__ void pass1b(const DevTables &T, const Stage<Spec<kind
                               : slot == 1 ? uses_nfw<kind_of_kv(KIND)>(1, fs) : uses_nfw<kind_of_kv(KIND)>(2, fs);
   if (!need || g >= ng || St.slot[g] == 0xffffffffu) return;
   const MassC &e = C.m[slot][g][c];
-  Mass M;
-  M.m = e.m;
-  M.L = e.L;
-  M.ir = e.ir;
-  M.it = e.it;
-  M.r200 = e.r200;
-  const Nfw nf = make_nfw(T, M, e.zt, f_of(St.args[g], fs));
+  const Nfw nf = make_nfw(T, mass_of(e), e.zt, f_of(St.args[g], fs));
   NfwC &en = C.nf[slot][g][c];
   en.c[fs] = nf.c;
   en.lc[fs] = nf.lc;
@@ -607,7 +611,11 @@ struct P2Layout {
   static constexpr int UPAD = pad(Cache<KIND>::NU_TOTAL);
   static constexpr int L3 = UPAD + (S::has_bilin ? pad(G * Cache<KIND>::NB) : 0);
   static constexpr int PK3 = L3 + pad(G * Cache<KIND>::N3);
-  static constexpr int END = PK3 + (S::has_pk ? pad(G * Cache<KIND>::N3Z) : 0);
+  // NsatNsat per (mass, z) class of its mass coordinate, where pass 1 runs
+  // in two stages (the occupations are then separate items)
+  static constexpr int NSS = PK3 + (S::has_pk ? pad(G * Cache<KIND>::N3Z) : 0);
+  static constexpr bool kNss = staged_p1<KIND>() && S::two_hods && S::nss_slot >= 0;
+  static constexpr int END = NSS + (kNss ? pad(G * 17) : 0);
 };
 
 // Pass 2 items after the u_NFW ones (i >= the u items' padded count).
@@ -619,6 +627,17 @@ __device__ __forceinline__ void pass2_rest(const DevTables &T, const Stage<Spec<
   constexpr int NU_TOTAL = Cache<KIND>::NU_TOTAL;
   constexpr int UPAD = (NU_TOTAL + 63) / 64 * 64;
   using Lay = P2Layout<KIND>;
+  if constexpr (Lay::kNss) {
+    if (i >= Lay::NSS) {
+      const int r = i - Lay::NSS;
+      if (r >= G * 17) return;
+      const int g = r / 17, c = r - g * 17;
+      if (g >= ng || St.slot[g] == 0xffffffffu) return;
+      MassC &e = C.m[S::nss_slot > 0 ? S::nss_slot : 0][g][c];
+      e.nss = nsatnsat<P>(St.args[g], e.ns1, e.ns2, mass_of(e));
+      return;
+    }
+  }
   if (i < Lay::L3) {
     if constexpr (S::has_bilin) {
       constexpr int NB = Cache<KIND>::NB;
