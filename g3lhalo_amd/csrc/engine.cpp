@@ -646,7 +646,8 @@ extern "C" {
 const char *g3l_version(void) { return "g3lhalo_amd 0.1 (gfx950)"; }
 
 int g3l_ctx_create(const g3l_tables *t, g3l_ctx **out) {
-  if (!t || !out || t->nbins < 2 || !t->g || !t->p_lens1 || !t->w || !t->dwdz || !t->hmf ||
+  if (!t || !out || t->nbins < 2 || t->nbins > 46340 || !t->g || !t->p_lens1 || !t->w ||
+      !t->dwdz || !t->hmf ||
       !t->P_lin || !t->b_h || !t->concentration)
     return G3L_EINVAL;
   if (!(t->zmax > t->zmin) || !(t->kmax > t->kmin) || !(t->mmax > t->mmin) || t->mmin <= 0 ||

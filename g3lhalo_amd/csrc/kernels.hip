@@ -177,11 +177,15 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 // Each cached value is computed by the same expressions from the same
 // coordinates as in DirectSrc, so the point values are bit-identical.
 
+// 72 bytes: it and zt (table indices below 2^16) share a word, so that a
+// record's stride across the LDS banks stays 18 dwords.
 struct MassC {
   double m, L, r200;
-  int ir, it, zt;
+  int ir;
+  uint16_t it, zt;
   double nc1, ns1, nc2, ns2, nss;
 };
+static_assert(sizeof(MassC) == 72, "MassC stride");
 
 // u_NFW normalisation (c, ln(1+c), F) per f selector (1, f1, f2).
 struct NfwC {
@@ -433,8 +437,8 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     e.L = M.L;
     e.r200 = M.r200;
     e.ir = M.ir;
-    e.it = M.it;
-    e.zt = zt;
+    e.it = (uint16_t)M.it;
+    e.zt = (uint16_t)zt;
     if constexpr (staged_p1<KIND>()) return;   // the rest in pass1b
     if constexpr (kind_of_kv(KIND) == kNN3) {
       // the NNMap 3-halo term needs the first HOD at m1 and the second at m2

@@ -49,8 +49,9 @@ typedef struct g3l_ctx g3l_ctx;
 /* Lookup tables and binning of NNMap_Model's constructor
  * (src/NNMap_Model.h:124-128, src/NMapMap_Model.h:102-106): 1-D arrays of
  * nbins values on z_i = zmin + i (zmax-zmin)/nbins, 2-D arrays z-major,
- * nbins*nbins values.  For NMapMap pass the one lens distribution as
- * p_lens1.  The context copies all tables into HBM. */
+ * nbins*nbins values (2 <= nbins <= 46340, so that nbins^2 indexes an
+ * int).  For NMapMap pass the one lens distribution as p_lens1.  The
+ * context copies all tables into HBM. */
 typedef struct g3l_tables {
   int32_t nbins;
   double zmin, zmax, kmin, kmax, mmin, mmax;

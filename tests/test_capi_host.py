@@ -38,6 +38,8 @@ def test_invalid_arguments_return_status():
     t = _lib.Tables()
     t.nbins = 1
     assert L.g3l_ctx_create(C.byref(t), C.byref(h)) == _lib.G3L_EINVAL
+    t.nbins = 46341   # nbins^2 would overflow the tables' int indices
+    assert L.g3l_ctx_create(C.byref(t), C.byref(h)) == _lib.G3L_EINVAL
     assert L.g3l_ctx_destroy(None) == _lib.G3L_EINVAL
     sid = C.c_int32()
     assert L.g3l_hodset_create(None, None, None, 0.0, 0.0, 0, C.byref(sid)) == _lib.G3L_EINVAL
