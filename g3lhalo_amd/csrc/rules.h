@@ -87,6 +87,26 @@ __device__ __forceinline__ void rule_point(int q, const double *c, const double 
   }
 }
 
+// s + p[0] + p[1] + ... + p[N-1], added in that order.  The values are
+// loaded in batches of 16 before they are added: left to the compiler, the
+// chain kept one LDS load in flight and waited a full LDS round trip for
+// every two additions.
+template <int N>
+__device__ __forceinline__ double ordered_sum(const double *p, double s) {
+  constexpr int B = 16;
+#pragma unroll
+  for (int k0 = 0; k0 < N; k0 += B) {
+    double buf[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      if (k0 + j < N) buf[j] = p[k0 + j];
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      if (k0 + j < N) s += buf[j];
+  }
+  return s;
+}
+
 // Genz-Malik reduction of one region's P values (src/hcubature.cpp:512-577).
 template <int D>
 __device__ __forceinline__ void gm_reduce(const double *v, double vol, double &val, double &err,
@@ -114,9 +134,9 @@ __device__ __forceinline__ void gm_reduce(const double *v, double vol, double &v
       arg = k;
     }
   }
-  const int k4 = 1 + 4 * D, n4 = 2 * D * (D - 1), k5 = k4 + n4;
-  for (int k = 0; k < n4; ++k) s4 += v[k4 + k];
-  for (int k = 0; k < (1 << D); ++k) s5 += v[k5 + k];
+  constexpr int k4 = 1 + 4 * D, n4 = 2 * D * (D - 1), k5 = k4 + n4;
+  s4 = ordered_sum<n4>(v + k4, s4);
+  s5 = ordered_sum<(1 << D)>(v + k5, s5);
   double r7 = vol * (w1 * f0 + w2 * s2 + w3 * s3 + w4 * s4 + w5 * s5);
   double r5 = vol * (e1 * f0 + e2 * s2 + e3 * s3 + e4 * s4);
   val = r7;
