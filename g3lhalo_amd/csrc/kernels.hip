@@ -242,6 +242,7 @@ struct Cache {
                 "seventeen (mass, z) classes");
   double l1[G][7], ap1[G][7], l2[G][7], ap2[G][7], cphi[G][7], z[G][7], w[G][7];
   double zpre[G][7], zfac[G][7];
+  double rw[G][7];   // 1/w, correctly rounded: k = l / w by div_rn
   // NNMap 3-halo: dwdz, 1/dwdz, g, p1, p2, 1/w per redshift (proj_3h)
   double ztab[kind_of_kv(KIND) == kNN3 ? G : 1][7][6];
   int zr[G][7], zt[G][7];
@@ -408,6 +409,7 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
     C.zr[g][c] = zr;
     C.zt[g][c] = zix_trunc(T, z);
     C.w[g][c] = w;
+    C.rw[g][c] = 1. / w;
     zfactors<kind_of_kv(KIND)>(T, St.args[g], zr, z, w, C.zpre[g][c], C.zfac[g][c]);
     if constexpr (kind_of_kv(KIND) == kNN3) {
       double *t = C.ztab[g][c];
@@ -582,7 +584,7 @@ __device__ __forceinline__ bool u_inputs(const DevTables &T,
       const double cp = C.cphi[g][kProj<D, bit(2)>.cls[q]];
       l = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cp);
     }
-    k = l / w;
+    k = div_rn(l, w, C.rw[g][cz]);   // = l / w (IEEE), in three instructions
     const int cm = kProj<D, bit(S::mdim[F::s.slot]) | bit(D - 1)>.cls[q];
     const MassC &e = C.m[F::s.slot][g][cm];
     if constexpr (cache_nfw<KIND>()) {
