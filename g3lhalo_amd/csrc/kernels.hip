@@ -490,18 +490,22 @@ __device__ __forceinline__ void pass1(const DevTables &T, const Stage<Spec<kind_
 }
 
 // Second stage of pass 1 (staged_p1 kinds): item i is one of a (mass, z)
-// class's HOD occupations (the first HOD's, and the second's for the
-// two-HOD kinds: NM x G x 17 items each) or one of its NFW normalisations (3
-// x NM x G x 17 items, one per f selector), from the masses of the first
-// stage.  NsatNsat, which needs both occupations, is a pass-2 item.
+// class's NFW normalisations (3 x NM x G x 17 items, one per f selector) or
+// HOD occupations (the first HOD's, and the second's for the two-HOD kinds:
+// NM x G x 17 items each), from the masses of the first stage, or l3 with its
+// aperture filter per (l1, l2, phi) class (G x N3 items), from the first
+// stage's l1, l2 and cos phi.  NsatNsat, which needs both occupations, is a
+// pass-2 item.  The normalisations come first: where a thread takes a second
+// item, its first is one of the shortest.
 template <int KIND>
 struct P1bLayout {
   using S = Spec<kind_of_kv(KIND)>;
   static constexpr int G = KGroup<KIND>::G;
   static constexpr int pad(int n) { return (n + 63) / 64 * 64; }
   static constexpr int NMZ = S::NM * G * 17;
-  static constexpr int OCC2 = pad(NMZ), NFW = OCC2 + (S::two_hods ? pad(NMZ) : 0);
-  static constexpr int END = NFW + pad(3 * NMZ);
+  static constexpr int OCC1 = pad(3 * NMZ), OCC2 = OCC1 + pad(NMZ);
+  static constexpr int L3 = OCC2 + (S::two_hods ? pad(NMZ) : 0);
+  static constexpr int END = L3 + pad(G * ProjTab<S::D, kL3>::n);
 };
 
 __device__ __forceinline__ Mass mass_of(const MassC &e) {
@@ -519,10 +523,25 @@ __device__ __forceinline__ void pass1b(const DevTables &T, const Stage<Spec<kind
                                        Cache<KIND> &C, int i, int ng) {
   using S = Spec<kind_of_kv(KIND)>;
   using Lay = P1bLayout<KIND>;
-  constexpr int G = KGroup<KIND>::G;
-  if (i < Lay::NFW) {
+  constexpr int D = S::D, G = KGroup<KIND>::G;
+  if (i < Lay::OCC1) {
+    if (i >= 3 * Lay::NMZ) return;
+    const int fs = i / Lay::NMZ, rm = i - fs * Lay::NMZ;
+    const int slot = rm / (G * 17), rr = rm - slot * G * 17, g = rr / 17, c = rr - g * 17;
+    const bool need = slot == 0 ? uses_nfw<kind_of_kv(KIND)>(0, fs)
+                                : slot == 1 ? uses_nfw<kind_of_kv(KIND)>(1, fs) : uses_nfw<kind_of_kv(KIND)>(2, fs);
+    if (!need || g >= ng || St.slot[g] == 0xffffffffu) return;
+    const MassC &e = C.m[slot][g][c];
+    const Nfw nf = make_nfw(T, mass_of(e), e.zt, f_of(St.args[g], fs));
+    NfwC &en = C.nf[slot][g][c];
+    en.c[fs] = nf.c;
+    en.lc[fs] = nf.lc;
+    en.F[fs] = nf.F;
+    return;
+  }
+  if (i < Lay::L3) {
     const bool second = i >= Lay::OCC2;
-    const int r = second ? i - Lay::OCC2 : i;
+    const int r = i - (second ? Lay::OCC2 : Lay::OCC1);
     if (r >= Lay::NMZ) return;
     const int slot = r / (G * 17), rr = r - slot * G * 17, g = rr / 17, c = rr - g * 17;
     if (g >= ng || St.slot[g] == 0xffffffffu) return;
@@ -540,19 +559,17 @@ __device__ __forceinline__ void pass1b(const DevTables &T, const Stage<Spec<kind
     }
     return;
   }
-  const int r = i - Lay::NFW;
-  if (r >= 3 * Lay::NMZ) return;
-  const int fs = r / Lay::NMZ, rm = r - fs * Lay::NMZ;
-  const int slot = rm / (G * 17), rr = rm - slot * G * 17, g = rr / 17, c = rr - g * 17;
-  const bool need = slot == 0 ? uses_nfw<kind_of_kv(KIND)>(0, fs)
-                              : slot == 1 ? uses_nfw<kind_of_kv(KIND)>(1, fs) : uses_nfw<kind_of_kv(KIND)>(2, fs);
-  if (!need || g >= ng || St.slot[g] == 0xffffffffu) return;
-  const MassC &e = C.m[slot][g][c];
-  const Nfw nf = make_nfw(T, mass_of(e), e.zt, f_of(St.args[g], fs));
-  NfwC &en = C.nf[slot][g][c];
-  en.c[fs] = nf.c;
-  en.lc[fs] = nf.lc;
-  en.F[fs] = nf.F;
+  // l3 and its filter per (l1, l2, phi) class, the expressions of the Geo
+  constexpr int N3 = Cache<KIND>::N3;
+  const int r = i - Lay::L3;
+  if (r >= G * N3) return;
+  const int g = r / N3, c = r - g * N3;
+  if (g >= ng || St.slot[g] == 0xffffffffu) return;
+  const int q = kProj<D, kL3>.rep[c];
+  const double l1 = C.l1[g][kProj<D, kL1>.cls[q]], l2 = C.l2[g][kProj<D, kL2>.cls[q]];
+  const double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * C.cphi[g][kProj<D, bit(2)>.cls[q]]);
+  C.l3[g][c] = l3;
+  C.ap3[g][c] = P::ap(St.args[g].t3 * l3);
 }
 
 // Pass 2: the inputs of u-factor item i (factor I or later).
@@ -579,6 +596,8 @@ __device__ __forceinline__ bool u_inputs(const DevTables &T,
       l = C.l1[g][kProj<D, kL1>.cls[q]];
     } else if constexpr (F::s.kidx == 2) {
       l = C.l2[g][kProj<D, kL2>.cls[q]];
+    } else if constexpr (staged_p1<KIND>()) {
+      l = C.l3[g][kProj<D, kL3>.cls[q]];   // from pass 1, the same expression
     } else {
       const double l1 = C.l1[g][kProj<D, kL1>.cls[q]], l2 = C.l2[g][kProj<D, kL2>.cls[q]];
       const double cp = C.cphi[g][kProj<D, bit(2)>.cls[q]];
@@ -616,7 +635,7 @@ struct P2Layout {
   static constexpr int pad(int n) { return (n + 63) / 64 * 64; }
   static constexpr int UPAD = pad(Cache<KIND>::NU_TOTAL);
   static constexpr int L3 = UPAD + (S::has_bilin ? pad(G * Cache<KIND>::NB) : 0);
-  static constexpr int PK3 = L3 + pad(G * Cache<KIND>::N3);
+  static constexpr int PK3 = L3 + (staged_p1<KIND>() ? 0 : pad(G * Cache<KIND>::N3));
   // NsatNsat per (mass, z) class of its mass coordinate, where pass 1 runs
   // in two stages (the occupations are then separate items)
   static constexpr int NSS = PK3 + (S::has_pk ? pad(G * Cache<KIND>::N3Z) : 0);
@@ -682,8 +701,13 @@ __device__ __forceinline__ void pass2_rest(const DevTables &T, const Stage<Spec<
     if (g >= ng || St.slot[g] == 0xffffffffu) return;
     const int q = kProj<D, kL3 | bit(D - 1)>.rep[c];
     const int cz = kProj<D, bit(D - 1)>.cls[q];
-    const double l1 = C.l1[g][kProj<D, kL1>.cls[q]], l2 = C.l2[g][kProj<D, kL2>.cls[q]];
-    const double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * C.cphi[g][kProj<D, bit(2)>.cls[q]]);
+    double l3;
+    if constexpr (staged_p1<KIND>()) {
+      l3 = C.l3[g][kProj<D, kL3>.cls[q]];
+    } else {
+      const double l1 = C.l1[g][kProj<D, kL1>.cls[q]], l2 = C.l2[g][kProj<D, kL2>.cls[q]];
+      l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * C.cphi[g][kProj<D, bit(2)>.cls[q]]);
+    }
     C.pk3[g][c] = pk(T, C.zr[g][cz], l3 / C.w[g][cz]);
   }
 }
