@@ -333,8 +333,9 @@ struct Occ {
 // or a negligible opposite-sign tail, so the result stays within a few
 // units in the last place of the reference's left-to-right form.
 
-// Si(x) and the rational part of Ci(x) = gamma + ln x + cr for x < 4.
-G3L_HD void sici_small_parts(double x, double &si, double &cr) {
+// Si(x) = sn / sd and the rational part of Ci(x) = gamma + ln x + cn / cd
+// for x < 4, as numerators and denominators (sd, cd >= 1).
+G3L_HD void sici_small_nd(double x, double &sn, double &sd, double &cn, double &cd) {
   const double z = x * x;
   const double z2 = z * z, z3 = z2 * z, z4 = z2 * z2;
   const double z5 = z4 * z, z6 = z3 * z3, z7 = z4 * z3;
@@ -351,7 +352,8 @@ G3L_HD void sici_small_parts(double x, double &si, double &cr) {
   b = fma(3.28067571055789734e-10, z4, b);
   b = fma(4.5049097575386581e-13, z5, b);
   b = fma(3.21107051193712168e-16, z6, b);
-  si = fdiv(x * a, b);
+  sn = x * a;
+  sd = b;
   // x^2 times Ci's numerator, -x^2/4 + ..., summed from its first term
   double n = -0.25 * z;
   n = fma(7.51851524438898291e-3, z2, n);
@@ -367,25 +369,34 @@ G3L_HD void sici_small_parts(double x, double &si, double &cr) {
   d = fma(1.38536352772778619e-12, z5, d);
   d = fma(1.89106054713059759e-15, z6, d);
   d = fma(1.39759616731376855e-18, z7, d);
-  cr = fdiv(n, d);
+  cn = n;
+  cd = d;
+}
+// Si(x) and the rational part of Ci(x) = gamma + ln x + cr for x < 4.
+G3L_HD void sici_small_parts(double x, double &si, double &cr) {
+  double sn, sd, cn, cd;
+  sici_small_nd(x, sn, sd, cn, cd);
+  si = fdiv(sn, sd);
+  cr = fdiv(cn, cd);
 }
 
-// g(x) and f(x) - 1/x for x >= 4, the latter from the exact coefficient
+// g(x) = gn / gd and f(x) - 1/x = fn / fd for x >= 4, as numerators and
+// denominators (gd >= 1, fd >= x), the latter from the exact coefficient
 // differences of f's numerator and denominator (no cancellation); t = 1/x^2.
-G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
+G3L_HD void sici_aux_nd(double x, double &gn, double &gd, double &fn, double &fd) {
   const double t = fdiv(1.0, x * x);
   const double t2 = t * t, t3 = t2 * t, t4 = t2 * t2, t5 = t4 * t;
   const double t6 = t3 * t3, t7 = t4 * t3, t8 = t4 * t4, t9 = t5 * t4, t10 = t5 * t5;
-  // 1 + t fd(t): f's denominator
-  double fd = fma(7.46437068161927678031e2, t, 1.0);
-  fd = fma(1.97865247031583951450e5, t2, fd);
-  fd = fma(2.41535670165126845144e7, t3, fd);
-  fd = fma(1.47478952192985464958e9, t4, fd);
-  fd = fma(4.58595115847765779830e10, t5, fd);
-  fd = fma(7.08501308149515401563e11, t6, fd);
-  fd = fma(5.06084464593475076774e12, t7, fd);
-  fd = fma(1.43468549171581016479e13, t8, fd);
-  fd = fma(1.11535493509914254097e13, t9, fd);
+  // fq = 1 + O(t): f's denominator
+  double fq = fma(7.46437068161927678031e2, t, 1.0);
+  fq = fma(1.97865247031583951450e5, t2, fq);
+  fq = fma(2.41535670165126845144e7, t3, fq);
+  fq = fma(1.47478952192985464958e9, t4, fq);
+  fq = fma(4.58595115847765779830e10, t5, fq);
+  fq = fma(7.08501308149515401563e11, t6, fq);
+  fq = fma(5.06084464593475076774e12, t7, fq);
+  fq = fma(1.43468549171581016479e13, t8, fq);
+  fq = fma(1.11535493509914254097e13, t9, fq);
   // t dn(t): numerator minus denominator of f, times t
   double dn = -1.99999999999097743952e0 * t;
   dn = fma(-1.46887413643708168820e3, t2, dn);
@@ -397,28 +408,37 @@ G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
   dn = fma(-4.26733661912124414062e12, t8, dn);
   dn = fma(-6.20538246899190625000e12, t9, dn);
   dn = fma(-4.94701168645415954590e11, t10, dn);
-  // 1 + t gn(t) and 1 + t gd(t): g's numerator and denominator
-  double gn = fma(8.1359520115168615e2, t, 1.0);
-  gn = fma(2.35239181626478200e5, t2, gn);
-  gn = fma(3.12557570795778731e7, t3, gn);
-  gn = fma(2.06297595146763354e9, t4, gn);
-  gn = fma(6.83052205423625007e10, t5, gn);
-  gn = fma(1.09049528450362786e12, t6, gn);
-  gn = fma(7.57664583257834349e12, t7, gn);
-  gn = fma(1.81004487464664575e13, t8, gn);
-  gn = fma(6.43291613143049485e12, t9, gn);
-  gn = fma(-1.36517137670871689e12, t10, gn);
-  double gd = fma(8.19595201151451564e2, t, 1.0);
-  gd = fma(2.40036752835578777e5, t2, gd);
-  gd = fma(3.26026661647090822e7, t3, gd);
-  gd = fma(2.23355543278099360e9, t4, gd);
-  gd = fma(7.87465017341829930e10, t5, gd);
-  gd = fma(1.39866710696414565e12, t6, gd);
-  gd = fma(1.17164723371736605e13, t7, gd);
-  gd = fma(4.01839087307656620e13, t8, gd);
-  gd = fma(3.99653257887490811e13, t9, gd);
-  g = fdiv(gn * t, gd);
-  fm = fdiv(dn, fd * x);
+  // gp, gq = 1 + O(t): g's numerator and denominator
+  double gp = fma(8.1359520115168615e2, t, 1.0);
+  gp = fma(2.35239181626478200e5, t2, gp);
+  gp = fma(3.12557570795778731e7, t3, gp);
+  gp = fma(2.06297595146763354e9, t4, gp);
+  gp = fma(6.83052205423625007e10, t5, gp);
+  gp = fma(1.09049528450362786e12, t6, gp);
+  gp = fma(7.57664583257834349e12, t7, gp);
+  gp = fma(1.81004487464664575e13, t8, gp);
+  gp = fma(6.43291613143049485e12, t9, gp);
+  gp = fma(-1.36517137670871689e12, t10, gp);
+  double gq = fma(8.19595201151451564e2, t, 1.0);
+  gq = fma(2.40036752835578777e5, t2, gq);
+  gq = fma(3.26026661647090822e7, t3, gq);
+  gq = fma(2.23355543278099360e9, t4, gq);
+  gq = fma(7.87465017341829930e10, t5, gq);
+  gq = fma(1.39866710696414565e12, t6, gq);
+  gq = fma(1.17164723371736605e13, t7, gq);
+  gq = fma(4.01839087307656620e13, t8, gq);
+  gq = fma(3.99653257887490811e13, t9, gq);
+  gn = gp * t;
+  gd = gq;
+  fn = dn;
+  fd = fq * x;
+}
+// g(x) and f(x) - 1/x for x >= 4.
+G3L_HD void sici_aux_gfm(double x, double &g, double &fm) {
+  double gn, gd, fn, fd;
+  sici_aux_nd(x, gn, gd, fn, fd);
+  g = fdiv(gn, gd);
+  fm = fdiv(fn, fd);
 }
 
 // ======================================================================
@@ -466,6 +486,55 @@ G3L_HD double u_nfw_a1(double a1, const Nfw &n) {
       (c1 - 1) * n.lc + fdiv(a1 * n.c - sx, a2) + c1 * (cr2 - cr1) + s1 * (si2 - si1);
   return 1 + fdiv(rest, n.F);
 }
+// u_nfw_a1 with each branch's quotients over one common denominator, so
+// that one division replaces six (a2 < 4: Si and Cr at a1 and a2,
+// (a1 c - sin a1 c) / a2 and / F), five (a1 < 4 <= a2: Si and Cr at a1, g
+// and f - 1/x at a2, / F) or four (a1 >= 4: g at a1, g and f - 1/x at a2,
+// / F).  Equal to u_nfw_a1 in exact arithmetic; the denominators are
+// products of the approximants' denominator polynomials (each >= 1 and below
+// 3e5 for these arguments) with a2 and F, so nothing overflows and no new
+// cancellation appears.  With a2 < 4, over D = a2 Cd1 Cd2 Sd1 Sd2 F:
+//   u = 1 + { [a2 (cos a1 - 1) ln(1+c) + a1 c - sin a1 c] Cd1 Cd2 Sd1 Sd2
+//             + a2 [cos a1 (Cn2 Cd1 - Cn1 Cd2) Sd1 Sd2
+//                   + sin a1 (Sn2 Sd1 - Sn1 Sd2) Cd1 Cd2] } / D.
+template <class P>
+G3L_HD double u_nfw_a1_1q(double a1, const Nfw &n) {
+  const double a2 = a1 * (1 + n.c);
+  double sx, cx;
+  sincos_cw(a1 * n.c, sx, cx);
+  double s1 = 0, c1 = 0, sn1 = 0, sd1 = 1, cn1 = 0, cd1 = 1;
+  double gn2 = 0, gd2 = 1, fn2 = 0, fd2 = 1;
+  if (a1 < 4) {
+    sincos_cw(a1, s1, c1);
+    sici_small_nd(a1, sn1, sd1, cn1, cd1);
+  }
+  if (a2 >= 4) sici_aux_nd(a2, gn2, gd2, fn2, fd2);
+  if (a1 >= 4) {
+    // u F = g(a1) + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c)
+    double gn1, gd1, fn1, fd1;
+    sici_aux_nd(a1, gn1, gd1, fn1, fd1);
+    const double p2 = fd2 * gd2;
+    const double r2 = sx * fn2 * gd2 - cx * gn2 * fd2;
+    return fdiv(fma(gn1, p2, gd1 * r2), gd1 * p2 * n.F);
+  }
+  if (a2 >= 4) {
+    // u F = pi/2 sin a1 - cos a1 (gamma + ln a1) - sin a1 Si(a1) - cos a1 Cr(a1)
+    //       + sin(a1 c) (f(a2) - 1/a2) - g(a2) cos(a1 c)
+    const double A = 0.5 * 3.14159265358979323846 * s1 - c1 * (0.5772156649 + P::ln_ci(a1));
+    const double p1 = sd1 * cd1, p2 = fd2 * gd2;
+    const double r1 = s1 * sn1 * cd1 + c1 * cn1 * sd1;
+    const double r2 = sx * fn2 * gd2 - cx * gn2 * fd2;
+    return fdiv(fma(A, p1, -r1) * p2 + r2 * p1, p1 * p2 * n.F);
+  }
+  double sn2, sd2, cn2, cd2;
+  sici_small_nd(a2, sn2, sd2, cn2, cd2);
+  const double C = cd1 * cd2, S = sd1 * sd2, CS = C * S;
+  const double dC = fma(cn2, cd1, -cn1 * cd2);
+  const double dS = fma(sn2, sd1, -sn1 * sd2);
+  const double t0 = fma((c1 - 1) * n.lc, a2, a1 * n.c - sx);
+  const double num = fma(t0, CS, a2 * fma(c1 * dC, S, s1 * dS * C));
+  return 1 + fdiv(num, a2 * CS * n.F);
+}
 // a1 = k r200 / c of u_NFW.
 G3L_HD double nfw_a1(const Nfw &n, double k) { return fdiv(k * n.r200, n.c); }
 template <class P>
@@ -507,8 +576,8 @@ struct P64 {
   }
 };
 
-G3L_HD double P64::u(const Nfw &n, double k) { return u_nfw<P64>(n, k); }
-G3L_HD double P64::u_a1(double a1, const Nfw &n) { return u_nfw_a1<P64>(a1, n); }
+G3L_HD double P64::u(const Nfw &n, double k) { return u_nfw_a1_1q<P64>(nfw_a1(n, k), n); }
+G3L_HD double P64::u_a1(double a1, const Nfw &n) { return u_nfw_a1_1q<P64>(a1, n); }
 
 // FP64 with the reference's own expressions for 10^x of l1, l2 and the r_200
 // cube root (pow), the HOD occupations and NsatNsat's power
@@ -547,18 +616,22 @@ struct P64R : P64 {
 
 // u_NFW for a2 < 4 (small k r_s), the FP64 branch of P64::u on its own:
 //   u = 1 + [(cos a1 - 1) ln(1+c) + (a1 c - sin a1 c)/a2
-//            + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si a2 - Si a1)] / F.
-// The Si/Ci differences of nearby arguments and the 1 - O(a1^2) structure
-// need FP64 here; equal to P64::u bit for bit in this branch.
+//            + cos a1 (Cr(a2) - Cr(a1)) + sin a1 (Si a2 - Si a1)] / F,
+// over one common denominator as in u_nfw_a1_1q.  The Si/Ci differences of
+// nearby arguments and the 1 - O(a1^2) structure need FP64 here; equal to
+// P64::u bit for bit in this branch.
 G3L_HD double u_small64(const Nfw &n, double a1, double a2) {
-  double sx, cx, s1, c1, si1, cr1, si2, cr2;
+  double sx, cx, s1, c1, sn1, sd1, cn1, cd1, sn2, sd2, cn2, cd2;
   sincos_cw(a1 * n.c, sx, cx);
   sincos_cw(a1, s1, c1);
-  sici_small_parts(a1, si1, cr1);
-  sici_small_parts(a2, si2, cr2);
-  const double rest =
-      (c1 - 1) * n.lc + fdiv(a1 * n.c - sx, a2) + c1 * (cr2 - cr1) + s1 * (si2 - si1);
-  return 1 + fdiv(rest, n.F);
+  sici_small_nd(a1, sn1, sd1, cn1, cd1);
+  sici_small_nd(a2, sn2, sd2, cn2, cd2);
+  const double C = cd1 * cd2, S = sd1 * sd2, CS = C * S;
+  const double dC = fma(cn2, cd1, -cn1 * cd2);
+  const double dS = fma(sn2, sd1, -sn1 * sd2);
+  const double t0 = fma((c1 - 1) * n.lc, a2, a1 * n.c - sx);
+  const double num = fma(t0, CS, a2 * fma(c1 * dC, S, s1 * dS * C));
+  return 1 + fdiv(num, a2 * CS * n.F);
 }
 
 // ======================================================================
