@@ -114,25 +114,41 @@ __device__ __forceinline__ int stage_regions(const EvalLaunch &L, Stage<D, G> &S
   return ng;
 }
 
-// Rule reduction of the block's regions from the point values in f.
+// Rule reduction of the block's regions from the point values in f.  For the
+// Genz-Malik rule lanes 2g and 2g + 1 of the first wave sum region g's 4th
+// and 5th point groups (gm_group_sum), and lane 2g finishes the region.
 template <int D, int G, int NP>
 __device__ __forceinline__ void reduce_regions(const EvalLaunch &L, const Stage<D, G> &S,
                                                const double *f, int ng) {
   const int tid = threadIdx.x;
-  if (tid < ng && S.slot[tid] != 0xffffffffu) {
-    double val, err;
-    int split = 0;
-    const double *v = f + tid * NP;
-    if constexpr (D == 1) {
-      gk_reduce(v, S.h[tid][0], val, err);
-    } else {
-      gm_reduce<D>(v, S.vol[tid], val, err, split);
+  const uint64_t St = L.stride;
+  if constexpr (D == 1) {
+    if (tid < ng && S.slot[tid] != 0xffffffffu) {
+      double val, err;
+      gk_reduce(f + tid * NP, S.h[tid][0], val, err);
+      const uint32_t sl = S.slot[tid];
+      L.geo[kFieldVal * St + sl] = val;
+      L.geo[kFieldErr * St + sl] = err;
+      L.split[sl] = 0;
     }
-    const uint32_t sl = S.slot[tid];
-    const uint64_t St = L.stride;
-    L.geo[kFieldVal * St + sl] = val;
-    L.geo[kFieldErr * St + sl] = err;
-    L.split[sl] = (uint8_t)split;
+  } else {
+    static_assert(2 * G <= 64, "a region's two lanes are in the first wave");
+    if (tid < 2 * ng) {
+      const int g = tid >> 1;
+      const bool fifth = tid & 1;
+      const double *v = f + g * NP;
+      const double s = gm_group_sum<D>(v, fifth);
+      const double s5 = __shfl_xor(s, 1);   // every lane below 2 ng is here
+      if (!fifth && S.slot[g] != 0xffffffffu) {
+        double val, err;
+        int split = 0;
+        gm_finish<D>(v, S.vol[g], s, s5, val, err, split);
+        const uint32_t sl = S.slot[g];
+        L.geo[kFieldVal * St + sl] = val;
+        L.geo[kFieldErr * St + sl] = err;
+        L.split[sl] = (uint8_t)split;
+      }
+    }
   }
 }
 

@@ -107,10 +107,30 @@ __device__ __forceinline__ double ordered_sum(const double *p, double s) {
   return s;
 }
 
-// Genz-Malik reduction of one region's P values (src/hcubature.cpp:512-577).
+// The sum of the 4th (2 D (D - 1) points, fifth false) or 5th (2^D points)
+// group of one region's P values, in the order of gm_reduce.  Both start with
+// their first min(2 D (D - 1), 2^D) terms, so that two lanes of a wave can
+// take the two sums of one region in one instruction stream; the longer
+// sum's lane then adds the rest alone.
 template <int D>
-__device__ __forceinline__ void gm_reduce(const double *v, double vol, double &val, double &err,
-                                          int &split) {
+__device__ __forceinline__ double gm_group_sum(const double *v, bool fifth) {
+  constexpr int k4 = 1 + 4 * D, n4 = 2 * D * (D - 1), n5 = 1 << D, k5 = k4 + n4;
+  constexpr int nc = n4 < n5 ? n4 : n5;
+  const double *p = v + (fifth ? k5 : k4);
+  double s = ordered_sum<nc>(p, 0.0);
+  if constexpr (n5 > n4) {
+    if (fifth) s = ordered_sum<n5 - nc>(p + nc, s);
+  } else if constexpr (n4 > n5) {
+    if (!fifth) s = ordered_sum<n4 - nc>(p + nc, s);
+  }
+  return s;
+}
+
+// Genz-Malik reduction of one region's P values (src/hcubature.cpp:512-577),
+// given the sums s4, s5 of its 4th and 5th point groups (gm_group_sum).
+template <int D>
+__device__ __forceinline__ void gm_finish(const double *v, double vol, double s4, double s5,
+                                          double &val, double &err, int &split) {
   const double w2 = 980. / 6561., w4 = 200. / 19683.;
   const double e2 = 245. / 486., e4 = 25. / 729.;
   const double ratio = (kLam2 * kLam2) / (kLam4 * kLam4);
@@ -120,7 +140,7 @@ __device__ __forceinline__ void gm_reduce(const double *v, double vol, double &v
   const double e1 = (double)(729 - 950 * D + 50 * D * D) / (double)729;
   const double e3 = (double)(265 - 100 * D) / (double)1458;
   double f0 = v[0];
-  double s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+  double s2 = 0, s3 = 0;
   double best = 0;
   int arg = 0;
 #pragma unroll
@@ -134,14 +154,16 @@ __device__ __forceinline__ void gm_reduce(const double *v, double vol, double &v
       arg = k;
     }
   }
-  constexpr int k4 = 1 + 4 * D, n4 = 2 * D * (D - 1), k5 = k4 + n4;
-  s4 = ordered_sum<n4>(v + k4, s4);
-  s5 = ordered_sum<(1 << D)>(v + k5, s5);
   double r7 = vol * (w1 * f0 + w2 * s2 + w3 * s3 + w4 * s4 + w5 * s5);
   double r5 = vol * (e1 * f0 + e2 * s2 + e3 * s3 + e4 * s4);
   val = r7;
   err = fabs(r5 - r7);
   split = arg;
+}
+template <int D>
+__device__ __forceinline__ void gm_reduce(const double *v, double vol, double &val, double &err,
+                                          int &split) {
+  gm_finish<D>(v, vol, gm_group_sum<D>(v, false), gm_group_sum<D>(v, true), val, err, split);
 }
 
 // Gauss-Kronrod 7/15 with the QUADPACK error scaling (src/hcubature.cpp:695-757).
