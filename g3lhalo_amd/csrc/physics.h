@@ -524,13 +524,13 @@ G3L_HD double u_nfw_a1(double a1, const Nfw &n) {
   return 1 + fdiv(rest, n.F);
 }
 // u_nfw_a1 with sin and cos by sincos_mm and each branch's quotients over
-// one common denominator, so that one division replaces six (a2 < 4: Si and Cr at a1 and a2,
-// (a1 c - sin a1 c) / a2 and / F), five (a1 < 4 <= a2: Si and Cr at a1, g
+// one common denominator, so that one division replaces six (a2 < 4: Si and
+// Cr at a1 and a2, (a1 c - sin a1 c) / a2 and / F), five (a1 < 4 <= a2: Si and Cr at a1, g
 // and f - 1/x at a2, / F) or four (a1 >= 4: g at a1, g and f - 1/x at a2,
 // / F).  Equal to u_nfw_a1 in exact arithmetic; the denominators are
-// products of the approximants' denominator polynomials (each >= 1 and below
-// 3e5 for these arguments) with a2 and F, so nothing overflows and no new
-// cancellation appears.  With a2 < 4, over D = a2 Cd1 Cd2 Sd1 Sd2 F:
+// products of the approximants' denominators (each at least 1, and at most
+// about 6e5 for arguments near 4 or x itself for large ones) with a2 and F,
+// so nothing overflows and no new cancellation appears.  With a2 < 4, over D = a2 Cd1 Cd2 Sd1 Sd2 F:
 //   u = 1 + { [a2 (cos a1 - 1) ln(1+c) + a1 c - sin a1 c] Cd1 Cd2 Sd1 Sd2
 //             + a2 [cos a1 (Cn2 Cd1 - Cn1 Cd2) Sd1 Sd2
 //                   + sin a1 (Sn2 Sd1 - Sn1 Sd2) Cd1 Cd2] } / D.
