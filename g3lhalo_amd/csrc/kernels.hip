@@ -87,10 +87,27 @@ struct Stage {
   TermArgs args[G];
 };
 
+// The group of G list entries (regions) this block evaluates.  A grid's
+// blocks are dealt to the eight XCDs in turn (block b to XCD b mod 8), each
+// XCD with its own L2, and neighbouring list entries mostly hold neighbouring
+// slots of every geometry field; so each XCD takes runs of C consecutive
+// groups (one run per XCD in turn) rather than every eighth group, and the
+// runs stay short enough that every XCD still sees a mix of the list's
+// integrals.  A bijection on [0, gridDim.x) whatever the dispatch order (the
+// tail beyond the last full round of runs keeps its order): the results do
+// not depend on it.
+__device__ __forceinline__ uint32_t block_group() {
+  constexpr uint32_t X = 8, C = 16;
+  const uint32_t b = blockIdx.x, full = gridDim.x / (X * C) * (X * C);
+  if (b >= full) return b;
+  const uint32_t x = b % X, i = b / X;
+  return ((i / C) * X + x) * C + i % C;
+}
+
 template <int D, int G, int NT>
 __device__ __forceinline__ int stage_regions(const EvalLaunch &L, Stage<D, G> &S) {
   const int tid = threadIdx.x;
-  const uint32_t first = blockIdx.x * G;
+  const uint32_t first = block_group() * G;
   const int ng = (L.count - first) < (uint32_t)G ? (int)(L.count - first) : G;
   const uint64_t St = L.stride;
   if (tid < ng) {
@@ -163,7 +180,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   constexpr int NT = KGroup<KIND>::NT;
   __shared__ Stage<D, G> S;
   __shared__ double s_f[G * NP];
-  if (blockIdx.x * G >= L.count) return;
+  if (block_group() * G >= L.count) return;
   const int ng = stage_regions<D, G, NT>(L, S);
   const int tid = threadIdx.x;
   const DevTables T = L.T;
@@ -855,7 +872,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
   __shared__ double s_f[NF];
   __shared__ unsigned long long s_ucnt;   // u items per branch, 21-bit fields
   __shared__ uint16_t s_unf[kSort ? NU : 1];
-  if (blockIdx.x * G >= L.count) return;
+  if (block_group() * G >= L.count) return;
   const int ng = stage_regions<D, G, NT>(L, St);
   const int tid = threadIdx.x;
   const DevTables T = L.T;
@@ -1418,10 +1435,14 @@ __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
   //    the per-thread counts (carried across chunks in registers), and the
   //    chunk's selected slots are listed in LDS in rank order, so that
   //    consecutive lanes cut consecutive regions (contiguous child slots)
-  //    and each cut reads its region's geometry before writing any.
+  //    and each cut reads its region's geometry before writing any.  The
+  //    list holds the K halves that keep their parents' slots first and the
+  //    K new slots after them, both in slot order, so that the regions an
+  //    evaluation block stages sit in neighbouring slots of each field.
   __shared__ uint32_t s_selp[4 * kSelThreads];
   const uint64_t T = s_prefix;
   const uint32_t r = s_r;
+  const uint32_t K = s_K;
   const int D = d.dim;
   EvalItem *list = L.lists[d.kind] + s_listbase;
   uint64_t tie_carry = 0, sel_carry = 0;
@@ -1487,8 +1508,8 @@ __global__ __launch_bounds__(kSelThreads) void k_select(SelectLaunch L) {
       }
       L.geo[kFieldVol * S + p] = vol * 0.5;
       L.geo[kFieldVol * S + q] = vq;
-      list[2 * a] = EvalItem{p, (uint32_t)J};
-      list[2 * a + 1] = EvalItem{q, (uint32_t)J};
+      list[a] = EvalItem{p, (uint32_t)J};
+      list[K + a] = EvalItem{q, (uint32_t)J};
     }
     tie_carry += tot_t;
     sel_carry += tot_s2;
