@@ -37,16 +37,16 @@ KIND_NAMES = {0: "n_bar", 1: "NNMap 1h", 2: "NNMap 2h", 3: "NNMap 3h", 4: "NMapM
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this workload scaled by a copy kernel of known size
-# (profiles/k_eval_nnmap550_counters.md): 634.0 MB read + 37.2 MB written per
+# (profiles/k_eval_nnmap550_counters.md): 123.6 MB read + 14.6 MB written per
 # k_eval<NNMap 2h, FP64> launch of the nnmap550 step.
-TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 671262959}
+TRAFFIC_BYTES_PER_LAUNCH = {("nnmap550", 2): 138210534}
 
 # Hardware FP64 flops per evaluation, (kind, launch variant) -> 2 FMA + ADD +
 # MUL + TRANS of the SQ_INSTS_VALU_*_F64 counters x 64 lanes / evaluations,
 # from the same profile: the FP64 instructions the kernel issues (a lane
 # masked off in a branch still counts), against the credited algorithmic
 # work of flops.FLOPS_PER_EVAL.
-HW_FP64_FLOPS_PER_EVAL = {(1, 0): 775, (2, 3): 1111, (3, 2): 1079}
+HW_FP64_FLOPS_PER_EVAL = {(1, 0): 777, (2, 3): 1114, (3, 2): 1079}
 
 # Algorithmic HBM bytes per region an evaluation launch handles: the
 # region's centre, half-widths and volume (2D + 1 doubles) and its list item
